@@ -1,0 +1,1182 @@
+/*
+ * TEST INFRASTRUCTURE ONLY — the CPU oracle for the Griffon chemistry hot path.
+ *
+ * A plain FP64 C++ restatement of the reference's griffon algorithm, written from the
+ * reference's equations; used only by tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg, as the checker and as the timed CPU baseline. Never linked into,
+ * called by or shipped as the product (spitfire_amd/).
+ *
+ * Each function cites the reference file:line it restates (paths relative to the
+ * reference checkout, src/spitfire/griffon/...).
+ *
+ * Parity pins (see tests/): oracle/_ref (the reference griffon sources compiled
+ * unmodified) on identical inputs at ~1e-14; mass conservation sum(w)=0; central
+ * finite differences of the RHS for every Jacobian; LAPACK (numpy/scipy) for the
+ * block-tridiagonal factor/solve and the per-block eigenvalues.
+ */
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include <map>
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+enum CpKind { CP_UNKNOWN = 0, CP_CONST = 1, CP_NASA7 = 2 };
+enum RateKind { RK_SIMPLE = 1, RK_THIRD_BODY = 2, RK_LINDEMANN = 3, RK_TROE = 4 };
+enum KForm { KF_CONSTANT, KF_LINEAR, KF_QUADRATIC, KF_RECIPROCAL, KF_ARRHENIUS };
+enum TroeForm { TF_NONE, TF_T1, TF_T2, TF_T12, TF_T3, TF_T13, TF_T23, TF_T123 };
+enum Order { OR_ONE, OR_TWO, OR_ONE_ONE, OR_TWO_ONE, OR_ONE_TWO, OR_ONE_ONE_ONE, OR_OTHER };
+
+struct Rxn {
+  int type = RK_SIMPLE;
+  bool reversible = true;
+  bool hasOrders = false;
+  std::vector<int> rc_idx, rc_st;   // reactant stoich positive
+  std::vector<double> rc_invmw;
+  std::vector<int> pd_idx, pd_st;   // product stoich negative (chemistry_setup.cpp:391)
+  std::vector<double> pd_invmw;
+  std::vector<int> net_idx, net_st;
+  std::vector<double> net_mw;
+  std::vector<int> tb_idx;
+  std::vector<double> tb_eff;       // invmw * (eff - default)  (chemistry_setup.cpp:416)
+  std::vector<int> sp_idx;
+  std::vector<double> sp_ord, sp_invmw;
+  std::vector<char> sp_nz;
+  double baseEff = 0.;
+  double kf[3] = {0, 0, 0};
+  double kp[3] = {0, 0, 0};
+  double troe[4] = {0, 0, 0, 0};
+  int kForm = KF_ARRHENIUS;
+  int troeForm = TF_NONE;
+  int fwdOrder = OR_OTHER, revOrder = OR_OTHER;
+  int sumStoich = 0, sumRcStoich = 0, sumPdStoich = 0;
+};
+
+struct Mech {
+  int ns = 0;
+  double Ru = 8314.46261815324;
+  double pref = 101325.;
+  double Tref = 298.15;
+  std::vector<double> mw, invmw;
+  std::vector<int> cpkind;
+  std::vector<double> Tmin, Tmax;
+  std::vector<std::vector<double>> c;  // 15 coefficients per species, griffon packing
+  std::vector<Rxn> rx;
+};
+
+// chemistry_setup.cpp:460-732 — reaction finalization (net species, order codes, kForm, troeForm).
+void finalize(Rxn &r, const Mech &m) {
+  r.sumStoich = 0;
+  std::map<int, std::pair<int, double>> net;  // ordered by species index
+  for (size_t i = 0; i < r.rc_idx.size(); ++i) {
+    r.sumStoich += r.rc_st[i];
+    net[r.rc_idx[i]] = std::make_pair(r.rc_st[i], r.rc_invmw[i]);
+  }
+  for (size_t i = 0; i < r.pd_idx.size(); ++i) {
+    r.sumStoich += r.pd_st[i];
+    auto it = net.find(r.pd_idx[i]);
+    if (it == net.end())
+      net[r.pd_idx[i]] = std::make_pair(r.pd_st[i], r.pd_invmw[i]);
+    else
+      it->second.first += r.pd_st[i];
+  }
+  r.net_idx.clear(); r.net_st.clear(); r.net_mw.clear();
+  for (auto &kv : net) {
+    if (kv.second.first != 0) {
+      r.net_idx.push_back(kv.first);
+      r.net_st.push_back(kv.second.first);
+      r.net_mw.push_back(1. / kv.second.second);
+    }
+  }
+  const int nn = (int)r.net_idx.size();
+  if (nn < 2 || nn > 8) throw std::runtime_error("bad number of net reacting species");
+
+  auto order_code = [](const std::vector<int> &st, int sgn) {
+    const int n = (int)st.size();
+    auto s = [&](int i) { return st[i] * sgn; };
+    if (n == 1) return s(0) == 1 ? OR_ONE : (s(0) == 2 ? OR_TWO : OR_OTHER);
+    if (n == 2) {
+      if (s(0) == 1 && s(1) == 1) return OR_ONE_ONE;
+      if (s(0) == 1 && s(1) == 2) return OR_ONE_TWO;
+      if (s(0) == 2 && s(1) == 1) return OR_TWO_ONE;
+      return OR_OTHER;
+    }
+    if (n == 3) return (s(0) == 1 && s(1) == 1 && s(2) == 1) ? OR_ONE_ONE_ONE : OR_OTHER;
+    return OR_OTHER;
+  };
+  r.fwdOrder = order_code(r.rc_st, 1);
+  r.revOrder = order_code(r.pd_st, -1);
+  r.sumRcStoich = 0;
+  for (int s : r.rc_st) r.sumRcStoich += std::abs(s);
+  r.sumPdStoich = 0;
+  for (int s : r.pd_st) r.sumPdStoich += std::abs(s);
+
+  if (std::abs(r.kf[2]) < 1.e-6) {
+    if (std::abs(r.kf[1]) < 1.e-6) r.kForm = KF_CONSTANT;
+    else if (std::abs(r.kf[1] - 1) < 1.e-6) r.kForm = KF_LINEAR;
+    else if (std::abs(r.kf[1] - 2) < 1.e-6) r.kForm = KF_QUADRATIC;
+    else if (std::abs(r.kf[1] + 1) < 1.e-6) r.kForm = KF_RECIPROCAL;
+    else r.kForm = KF_ARRHENIUS;
+  } else {
+    r.kForm = KF_ARRHENIUS;
+  }
+  r.troeForm = TF_NONE;
+  if (r.type == RK_TROE) {
+    const bool t1 = std::abs(r.troe[1]) > 1.e-8, t2 = std::abs(r.troe[2]) > 1.e-8, t3 = std::abs(r.troe[3]) > 1.e-8;
+    if (t1) r.troeForm = t2 ? (t3 ? TF_T123 : TF_T12) : (t3 ? TF_T13 : TF_T1);
+    else r.troeForm = t2 ? (t3 ? TF_T23 : TF_T2) : (t3 ? TF_T3 : TF_NONE);
+  }
+  (void)m;
+}
+
+// combustion_kernels.h:381-387
+inline double mmw_of(const Mech &m, const double *y) {
+  double d = 0.;
+  for (int i = 0; i < m.ns; ++i) d += m.invmw[i] * y[i];
+  return 1. / d;
+}
+
+// combustion_kernels.h:526-530
+inline void extract_y(const Mech &m, const double *ynm1, double *y) {
+  y[m.ns - 1] = 1.;
+  for (int j = 0; j < m.ns - 1; ++j) {
+    y[j] = ynm1[j];
+    y[m.ns - 1] -= y[j];
+  }
+}
+
+inline double cp_poly7(const double *c, int o, double t) {  // c[o..o+4] high/low block
+  return c[o] + t * (2. * c[o + 1] + t * (6. * c[o + 2] + t * (12. * c[o + 3] + 20. * t * c[o + 4])));
+}
+
+// thermodynamics_kernels.cpp:45-131 (CONST and NASA7; clamped outside [Tmin,Tmax])
+void cp_mix_and_species(const Mech &m, double t, const double *y, double *cpmix, double *cpi) {
+  *cpmix = 0.;
+  for (int i = 0; i < m.ns; ++i) {
+    const double *c = m.c[i].data();
+    if (m.cpkind[i] == CP_CONST) {
+      cpi[i] = m.invmw[i] * c[3];
+    } else if (m.cpkind[i] == CP_NASA7) {
+      const double minT = m.Tmin[i], maxT = m.Tmax[i];
+      if (t <= c[0] && t >= minT) cpi[i] = m.invmw[i] * cp_poly7(c, 8, t);
+      else if (t > c[0] && t <= maxT) cpi[i] = m.invmw[i] * cp_poly7(c, 1, t);
+      else if (t < minT) cpi[i] = m.invmw[i] * cp_poly7(c, 8, minT);
+      else cpi[i] = m.invmw[i] * cp_poly7(c, 1, maxT);
+    } else {
+      continue;
+    }
+    *cpmix += y[i] * cpi[i];
+  }
+}
+
+// thermodynamics_kernels.cpp:183-260
+void cp_sens_T(const Mech &m, double t, const double *y, double *cpsens, double *cpisens) {
+  *cpsens = 0.;
+  for (int i = 0; i < m.ns; ++i) {
+    const double *c = m.c[i].data();
+    if (m.cpkind[i] == CP_CONST) {
+      cpisens[i] = 0.;
+      *cpsens = 0.;  // reproduces thermodynamics_kernels.cpp:202 (assignment, not accumulation)
+    } else if (m.cpkind[i] == CP_NASA7) {
+      if (t <= c[0] && t >= m.Tmin[i]) {
+        cpisens[i] = m.invmw[i] * ((2. * c[9] + t * (12. * c[10] + t * (36. * c[11] + 80. * t * c[12]))));
+        *cpsens += y[i] * cpisens[i];
+      } else if (t > c[0] && t <= m.Tmax[i]) {
+        cpisens[i] = m.invmw[i] * ((2. * c[2] + t * (12. * c[3] + t * (36. * c[4] + 80. * t * c[5]))));
+        *cpsens += y[i] * cpisens[i];
+      } else {
+        cpisens[i] = 0.;
+      }
+    }
+  }
+}
+
+// thermodynamics_kernels.cpp:262-351
+void species_enthalpies(const Mech &m, double T, double *h) {
+  for (int i = 0; i < m.ns; ++i) {
+    const double *c = m.c[i].data();
+    const double im = m.invmw[i];
+    if (m.cpkind[i] == CP_CONST) {
+      h[i] = im * (c[1] + c[3] * (T - c[0]));
+    } else if (m.cpkind[i] == CP_NASA7) {
+      const double minT = m.Tmin[i], maxT = m.Tmax[i];
+      if (T <= c[0] && T >= minT)
+        h[i] = im * (c[13] + T * (c[8] + T * (c[9] + T * (2. * c[10] + T * (3. * c[11] + T * 4. * c[12])))));
+      else if (T > c[0] && T <= maxT)
+        h[i] = im * (c[6] + T * (c[1] + T * (c[2] + T * (2. * c[3] + T * (3. * c[4] + T * 4. * c[5])))));
+      else if (T < minT)
+        h[i] = im * (c[13] + c[8] * T + minT * (2. * c[9] * T + minT * (3. * 2. * c[10] * T - c[9] + minT * (4. * 3. * c[11] * T - 2. * 2. * c[10] + minT * (5. * 4. * c[12] * T - 3. * 3. * c[11] + minT * -4. * 4. * c[12])))));
+      else
+        h[i] = im * (c[6] + c[1] * T + maxT * (2. * c[2] * T + maxT * (3. * 2. * c[3] * T - c[2] + maxT * (4. * 3. * c[4] * T - 2. * 2. * c[3] + maxT * (5. * 4. * c[5] * T - 3. * 3. * c[4] + maxT * -4. * 4. * c[5])))));
+    }
+  }
+}
+
+// chemistry_kernels.cpp:56-97 (specG, NOT clamped to [Tmin,Tmax]); rates_sensitivities_exact.cpp:82-126 adds dB/dT
+void species_gibbs(const Mech &m, double T, double logT, double invT, double *G, double *dBdT) {
+  const double invRu = 1. / m.Ru;
+  const double Ru = 1. / invRu;
+  for (int n = 0; n < m.ns; ++n) {
+    const double *c = m.c[n].data();
+    if (m.cpkind[n] == CP_NASA7) {
+      if (T <= c[0]) {
+        G[n] = c[13] + T * (c[8] - c[14] - c[8] * logT - T * (c[9] + T * (c[10] + T * (c[11] + T * c[12]))));
+        if (dBdT) dBdT[n] = invRu * ((c[8] - Ru) * invT + c[9] + T * (2 * c[10] + T * (3 * c[11] + T * 4 * c[12])) + c[13] * invT * invT);
+      } else {
+        G[n] = c[6] + T * (c[1] - c[7] - c[1] * logT - T * (c[2] + T * (c[3] + T * (c[4] + T * c[5]))));
+        if (dBdT) dBdT[n] = invRu * ((c[1] - Ru) * invT + c[2] + T * (2 * c[3] + T * (3 * c[4] + T * 4 * c[5])) + c[6] * invT * invT);
+      }
+    } else if (m.cpkind[n] == CP_CONST) {
+      G[n] = c[1] + c[3] * (T - c[0]) - T * (c[2] + c[3] * (logT - std::log(c[0])));
+      if (dBdT) dBdT[n] = invT * (m.mw[n] * invRu * (c[3] - invT * (c[3] * c[0] - c[1])) - 1);
+    } else {
+      throw std::runtime_error("unsupported thermo");
+    }
+  }
+}
+
+inline double arrh(const double *k, double logT, double invT) { return k[0] * std::exp(k[1] * logT - k[2] * invT); }
+inline double arrh_sens_over_k(const double *k, double invT) { return invT * (k[1] + k[2] * invT); }
+
+inline double kf_of(const Rxn &r, double T, double logT, double invT) {
+  switch (r.kForm) {
+    case KF_CONSTANT: return r.kf[0];
+    case KF_LINEAR: return r.kf[0] * T;
+    case KF_QUADRATIC: return r.kf[0] * T * T;
+    case KF_RECIPROCAL: return r.kf[0] * invT;
+    default: return arrh(r.kf, logT, invT);
+  }
+}
+
+// sum of tb_eff*y over third bodies in griffon's summation order (chemistry_kernels.cpp:159-199)
+// third-body concentration baseEff*conc + rho*sum(eff*y): the first eight terms are summed before the
+// multiplication by rho, later ones are added one at a time (chemistry_kernels.cpp:164-199)
+inline double tb_conc(const Rxn &r, const double *y, double conc, double rho) {
+  const size_t n = r.tb_idx.size();
+  if (n == 0) return r.baseEff * conc;
+  double s = 0.;
+  for (size_t i = 0; i < n && i < 8; ++i) s += r.tb_eff[i] * y[r.tb_idx[i]];
+  double M = r.baseEff * conc + rho * s;
+  for (size_t i = 8; i < n; ++i) M += rho * (r.tb_eff[i] * y[r.tb_idx[i]]);
+  return M;
+}
+
+inline double troe_fcent(const Rxn &r, double T, double invT) {
+  const double *t = r.troe;
+  switch (r.troeForm) {
+    case TF_T123: return (1 - t[0]) * std::exp(-T / t[1]) + t[0] * std::exp(-T / t[2]) + std::exp(-invT * t[3]);
+    case TF_T12: return (1 - t[0]) * std::exp(-T / t[1]) + t[0] * std::exp(-T / t[2]) + 0.0;
+    case TF_T1: return (1 - t[0]) * std::exp(-T / t[1]) + 0.0 + 0.0;
+    case TF_T23: return 0.0 + t[0] * std::exp(-T / t[2]) + std::exp(-invT * t[3]);
+    case TF_T2: return 0.0 + t[0] * std::exp(-T / t[2]) + 0.0;
+    case TF_T13: return (1 - t[0]) * std::exp(-T / t[1]) + 0.0 + std::exp(-invT * t[3]);
+    case TF_T3: return 0.0 + 0.0 + std::exp(-invT * t[3]);
+    default: throw std::runtime_error("no troe terms flagged for evaluation");
+  }
+}
+
+inline double mass_action(const std::vector<int> &idx, const std::vector<int> &st, const std::vector<double> &invmw,
+                          int order, const double *y, double rho, double k) {
+  auto C = [&](int i) { return y[idx[i]] * rho * invmw[i]; };
+  switch (order) {
+    case OR_ONE: return k * C(0);
+    case OR_TWO: return k * C(0) * C(0);
+    case OR_ONE_ONE: return k * C(0) * C(1);
+    case OR_ONE_ONE_ONE: return k * C(0) * C(1) * C(2);
+    case OR_TWO_ONE: return k * C(0) * C(0) * C(1);
+    case OR_ONE_TWO: return k * C(0) * C(1) * C(1);
+    default:
+      for (size_t i = 0; i < idx.size(); ++i) {
+        switch (std::abs(st[i])) {
+          case 1: k *= C(i); break;
+          case 2: k *= C(i) * C(i); break;
+          case 3: k *= C(i) * C(i) * C(i); break;
+        }
+      }
+      return k;
+  }
+}
+
+// chemistry_kernels.cpp:35-463 — net mass production rates
+void production_rates(const Mech &m, double T, double rho, double mmw, const double *y, double *w) {
+  const double invT = 1 / T;
+  const double logT = std::log(T);
+  const double conc = rho / mmw;
+  const double invRu = 1. / m.Ru;
+  const double port = m.pref * invT * invRu;
+  for (int i = 0; i < m.ns; ++i) w[i] = 0.;
+  std::vector<double> G(m.ns);
+  species_gibbs(m, T, logT, invT, G.data(), nullptr);
+
+  for (const Rxn &r : m.rx) {
+    double k = kf_of(r, T, logT, invT);
+    double kr = 0.;
+    switch (r.type) {
+      case RK_SIMPLE: break;
+      case RK_THIRD_BODY: {
+        const double M = tb_conc(r, y, conc, rho);
+        k *= M;
+        break;
+      }
+      case RK_LINDEMANN: {
+        const double M = tb_conc(r, y, conc, rho);
+        k /= (1 + k / (arrh(r.kp, logT, invT) * M));
+        break;
+      }
+      case RK_TROE: {
+        const double M = tb_conc(r, y, conc, rho);
+        const double pr = arrh(r.kp, logT, invT) / k * M;
+        const double logFCent = std::log10(troe_fcent(r, T, invT));
+        const double logPrC = std::log10(std::max(pr, 1.e-300)) + (-0.4 - 0.67 * logFCent);
+        const double F1 = logPrC / ((0.75 - 1.27 * logFCent) - 0.14 * logPrC);
+        k = k * std::pow(10, logFCent / (1 + F1 * F1)) * pr / (1 + pr);
+        break;
+      }
+    }
+    if (r.hasOrders) {
+      for (size_t i = 0; i < r.sp_idx.size(); ++i)
+        if (r.sp_nz[i]) k *= std::pow(std::max(y[r.sp_idx[i]] * rho * r.sp_invmw[i], 0.), r.sp_ord[i]);
+      kr = 0.;
+    } else {
+      if (r.reversible) {
+        double sg = 0.;
+        for (size_t i = 0; i < r.net_idx.size(); ++i) sg += r.net_st[i] * G[r.net_idx[i]];
+        kr = k * std::exp(r.sumStoich * std::log(port) - invT * invRu * sg);
+      }
+      k = mass_action(r.rc_idx, r.rc_st, r.rc_invmw, r.fwdOrder, y, rho, k);
+      if (r.reversible) kr = mass_action(r.pd_idx, r.pd_st, r.pd_invmw, r.revOrder, y, rho, kr);
+    }
+    for (size_t i = 0; i < r.net_idx.size(); ++i) w[r.net_idx[i]] -= r.net_st[i] * r.net_mw[i] * (k - kr);
+  }
+}
+
+// d(mass-action product)/dC_j * dC_j/dY_j for a reactant/product slot j (rates_sensitivities_exact.cpp:338-424, 614-710)
+// returns k * rho * invmw_j * d/dC_j(prod C_i^nu_i); invmw_used substitutes 1/M_last for the last-species branch.
+// Stoichiometric coefficients are limited to 1..3 (griffon's mass-action switches only handle 1..3).
+double mass_action_deriv(const std::vector<int> &idx, const std::vector<int> &st, const std::vector<double> &invmw,
+                         int order, const double *y, double rho, double k, int j, double invmw_used) {
+  auto C = [&](int i) { return y[idx[i]] * rho * invmw[i]; };
+  const double base = k * rho * invmw_used;
+  switch (order) {
+    case OR_ONE: return base;
+    case OR_TWO: return base * 2. * C(j);
+    case OR_ONE_ONE: return j == 0 ? base * C(1) : base * C(0);
+    case OR_ONE_ONE_ONE: return j == 0 ? base * C(1) * C(2) : (j == 1 ? base * C(0) * C(2) : base * C(0) * C(1));
+    case OR_TWO_ONE: return j == 0 ? base * 2. * C(0) * C(1) : base * C(0) * C(0);
+    case OR_ONE_TWO: return j == 0 ? base * C(1) * C(1) : base * 2. * C(1) * C(0);
+    default: {
+      double v = base;
+      const int sj = std::abs(st[j]);
+      if (sj == 2) v = base * 2. * C(j);
+      else if (sj == 3) v = base * 3. * C(j) * C(j);
+      for (size_t i = 0; i < idx.size(); ++i) {
+        if ((int)i == j) continue;
+        const int si = std::abs(st[i]);
+        if (si == 1) v *= C(i);
+        else if (si == 2) v *= C(i) * C(i);
+        else if (si == 3) v *= C(i) * C(i) * C(i);
+      }
+      return v;
+    }
+  }
+}
+
+// rates_sensitivities_exact.cpp:33-1028 — option-0 exact sensitivities, wsens[(ns+1)^2] column-major
+// (col 0 d/drho, col 1 d/dT, col 2+s d/dY_s; row = species; stride ns+1)
+void prod_rates_sens_exact(const Mech &m, double T, double rho, double Mmix, const double *y, double *w, double *ws) {
+  const int ns = m.ns;
+  const int nsp1 = ns + 1;
+  for (int i = 0; i < ns; ++i) w[i] = 0.;
+  for (int i = 0; i < nsp1 * nsp1; ++i) ws[i] = 0.;
+  std::vector<double> G(ns), dBdT(ns), dCdY(ns), dRdY(ns - 1), dqdY(ns - 1);
+  const double invT = 1. / T;
+  const double logT = std::log(T);
+  const double invM = 1. / Mmix;
+  const double ct = rho * invM;
+  const double invRu = 1. / m.Ru;
+  species_gibbs(m, T, logT, invT, G.data(), dBdT.data());
+  const double *invMsp = m.invmw.data();
+  double Kc = 0., dKcdToverKc = 0.;  // persist across reactions as in the reference (n_net > 6 keeps the previous value)
+
+  for (const Rxn &r : m.rx) {
+    double Rnet = 0., dRdrho = 0., dRdT = 0.;
+    double Ctbaf = 1., dCdrho = 0., dCdT = 0.;
+    for (int i = 0; i < ns - 1; ++i) { dRdY[i] = 0.; dCdY[i] = 0.; dqdY[i] = 0.; }
+    const double kf = kf_of(r, T, logT, invT);
+
+    if (r.hasOrders) {
+      auto CS = [&](int i) { return y[r.sp_idx[i]] * rho * r.sp_invmw[i]; };
+      double sumOrders = 0.;
+      Rnet = kf;
+      const int nsp = (int)r.sp_idx.size();
+      for (int i = 0; i < nsp; ++i)
+        if (r.sp_nz[i]) { Rnet *= std::pow(std::max(CS(i), 0.), r.sp_ord[i]); sumOrders += r.sp_ord[i]; }
+      dRdrho = Rnet / ct * invM * sumOrders;
+      dRdT = Rnet * arrh_sens_over_k(r.kf, invT);
+      int nsIdx = -1;
+      for (int j = 0; j < nsp; ++j) {
+        const int s = r.sp_idx[j];
+        if (s == ns - 1) { nsIdx = j; continue; }
+        if (!r.sp_nz[j]) continue;
+        double mod = kf;
+        for (int l = 0; l < nsp; ++l) {
+          if (l != j) {
+            if (r.sp_nz[l]) mod *= std::pow(std::max(CS(l), 0.), r.sp_ord[l]);
+          } else {
+            if (r.sp_ord[l] > 1) mod *= r.sp_ord[l] * rho * r.sp_invmw[l] * std::pow(std::max(CS(l), 1.e-16), r.sp_ord[l] - 1.);
+            else mod *= r.sp_ord[l] * rho * r.sp_invmw[l] / std::pow(std::max(CS(l), 1.e-16), 1. - r.sp_ord[l]);
+          }
+        }
+        dRdY[s] = mod;
+      }
+      if (nsIdx >= 0 && r.sp_nz[nsIdx]) {
+        double t = kf;
+        for (int i = 0; i < nsp; ++i) {
+          if (i != nsIdx) { if (r.sp_nz[i]) t *= std::pow(CS(i), r.sp_ord[i]); }
+          else t *= r.sp_ord[i] * rho * r.sp_invmw[i] * std::pow(std::max(CS(i), 1.e-16), r.sp_ord[i] - 1.);
+        }
+        for (int s = 0; s < ns - 1; ++s) dRdY[s] -= t;
+      }
+    } else {
+      Rnet = mass_action(r.rc_idx, r.rc_st, r.rc_invmw, r.fwdOrder, y, rho, kf);
+      dRdrho = Rnet / ct * invM * r.sumRcStoich;
+      dRdT = Rnet * arrh_sens_over_k(r.kf, invT);
+      int nsIdx = -1;
+      for (int j = 0; j < (int)r.rc_idx.size(); ++j) {
+        const int s = r.rc_idx[j];
+        if (s != ns - 1) dRdY[s] = mass_action_deriv(r.rc_idx, r.rc_st, r.rc_invmw, r.fwdOrder, y, rho, kf, j, r.rc_invmw[j]);
+        else nsIdx = j;
+      }
+      if (nsIdx >= 0) {
+        const double t = mass_action_deriv(r.rc_idx, r.rc_st, r.rc_invmw, r.fwdOrder, y, rho, kf, nsIdx, invMsp[ns - 1]);
+        for (int s = 0; s < ns - 1; ++s) dRdY[s] -= t;
+      }
+      if (r.reversible) {
+        const int nn = (int)r.net_idx.size();
+        if (nn >= 2 && nn <= 6) {
+          double sg = 0., sb = 0.;
+          for (int i = 0; i < nn; ++i) sg += r.net_st[i] * G[r.net_idx[i]];
+          for (int i = 0; i < nn; ++i) sb += r.net_st[i] * dBdT[r.net_idx[i]];
+          Kc = std::exp(-(r.sumStoich * std::log(m.pref * invT * invRu) - invT * invRu * sg));
+          dKcdToverKc = -sb;
+        }
+        const double kr = kf / Kc;
+        const double Rr = mass_action(r.pd_idx, r.pd_st, r.pd_invmw, r.revOrder, y, rho, kr);
+        Rnet -= Rr;
+        dRdrho -= Rr / ct * invM * r.sumPdStoich;
+        dRdT -= Rr * (arrh_sens_over_k(r.kf, invT) - dKcdToverKc);
+        int nsP = -1;
+        for (int j = 0; j < (int)r.pd_idx.size(); ++j) {
+          const int s = r.pd_idx[j];
+          if (s != ns - 1) dRdY[s] -= mass_action_deriv(r.pd_idx, r.pd_st, r.pd_invmw, r.revOrder, y, rho, kr, j, r.pd_invmw[j]);
+          else nsP = j;
+        }
+        if (nsP >= 0) {
+          double t;
+          if (r.revOrder == OR_OTHER && std::abs(r.pd_st[nsP]) == 3) {
+            // reproduces rates_sensitivities_exact.cpp:785 (`nsTmp *=` on a value carried over from earlier work);
+            // only reachable when the last species is a product with stoichiometry 3 — unsupported here.
+            throw std::runtime_error("last species as a product with stoichiometry 3 is not supported");
+          }
+          t = mass_action_deriv(r.pd_idx, r.pd_st, r.pd_invmw, r.revOrder, y, rho, kr, nsP, invMsp[ns - 1]);
+          for (int s = 0; s < ns - 1; ++s) dRdY[s] += t;
+        }
+      }
+    }
+
+    // third-body / falloff multiplier and its derivatives (rates_sensitivities_exact.cpp:817-1000)
+    if (r.type == RK_THIRD_BODY) {
+      Ctbaf = r.baseEff * ct;
+      for (size_t i = 0; i < r.tb_idx.size(); ++i) Ctbaf += rho * (r.tb_eff[i] * y[r.tb_idx[i]]);
+      dCdrho = r.baseEff * invM;
+      for (size_t i = 0; i < r.tb_idx.size(); ++i) dCdrho += (r.tb_eff[i] * y[r.tb_idx[i]]);
+      dCdT = 0.;
+      const double rb = rho * r.baseEff;
+      for (int s = 0; s < ns - 1; ++s) dCdY[s] = rb * (invMsp[s] - invMsp[ns - 1]);
+      for (size_t i = 0; i < r.tb_idx.size(); ++i) if (r.tb_idx[i] < ns - 1) dCdY[r.tb_idx[i]] += rho * r.tb_eff[i];
+      for (size_t i = 0; i < r.tb_idx.size(); ++i)
+        if (r.tb_idx[i] == ns - 1) { for (int s = 0; s < ns - 1; ++s) dCdY[s] -= rho * r.tb_eff[i]; break; }
+    } else if (r.type == RK_LINDEMANN || r.type == RK_TROE) {
+      double fCent = 0., dfCentdT = 0.;
+      if (r.type == RK_TROE) {
+        const double *t = r.troe;
+        const double t1 = std::exp(-T / t[1]), t2 = std::exp(-T / t[2]), t3 = std::exp(-invT * t[3]);
+        switch (r.troeForm) {
+          case TF_T123: fCent = (1 - t[0]) * t1 + t[0] * t2 + t3; dfCentdT = (t[0] - 1) / t[1] * t1 - t[0] / t[2] * t2 + t3 * t[3] * invT * invT; break;
+          case TF_T12: fCent = (1 - t[0]) * t1 + t[0] * t2; dfCentdT = (t[0] - 1) / t[1] * t1 - t[0] / t[2] * t2; break;
+          case TF_T1: fCent = (1 - t[0]) * t1; dfCentdT = (t[0] - 1) / t[1] * t1; break;
+          case TF_T23: fCent = t[0] * t2 + t3; dfCentdT = -t[0] / t[2] * t2 + t3 * t[3] * invT * invT; break;
+          case TF_T2: fCent = t[0] * t2; dfCentdT = -t[0] / t[2] * t2; break;
+          case TF_T13: fCent = (1 - t[0]) * t1 + t3; dfCentdT = (t[0] - 1) / t[1] * t1 + t3 * t[3] * invT * invT; break;
+          case TF_T3: fCent = t3; dfCentdT = t3 * t[3] * invT * invT; break;
+          default: throw std::runtime_error("no troe terms flagged for evaluation");
+        }
+      }
+      double flf = r.baseEff * ct;
+      for (size_t i = 0; i < r.tb_idx.size(); ++i) flf = flf + rho * (r.tb_eff[i] * y[r.tb_idx[i]]);
+      const double kpk = arrh(r.kp, logT, invT) / kf;
+      const double pr = kpk * flf;
+      const double dlnk = arrh_sens_over_k(r.kp, invT) - arrh_sens_over_k(r.kf, invT);
+      double nsTmp;
+      if (r.type == RK_LINDEMANN) {
+        Ctbaf = pr / (1. + pr);
+        dCdT = Ctbaf / (1. + pr) * dlnk;
+        nsTmp = kpk / ((1. + pr) * (1. + pr));
+      } else {
+        const double log10pr = std::log10(std::max(pr, 1.e-300));
+        const double log10fc = std::log10(std::max(fCent, 1.e-300));
+        const double logfc = std::log(std::max(fCent, 1.e-300));
+        const double ln10 = std::log(10.);
+        const double a = log10pr - 0.67 * log10fc - 0.4;
+        const double b = -0.14 * log10pr - 1.1762 * log10fc + 0.806;
+        const double g = 1 / (1 + (a / b) * (a / b));
+        const double fT = std::pow(fCent, g);
+        Ctbaf = fT * pr / (1 + pr);
+        const double dfTdT = fT * (g / fCent * dfCentdT + logfc * (-2.0 * g * g / ln10 * a / (b * b * b) * ((b + 0.14 * a) * dlnk - (0.67 * b - 1.1762 * a) * dfCentdT / fCent)));
+        dCdT = 1. / (1. + 1. / pr) * dfTdT + fT * pr / ((1. + pr) * (1. + pr)) * dlnk;
+        nsTmp = kpk * (-2.0 / (1. + pr) * fT * logfc * g * g / ln10 * a / (b * b * b) * (b + 0.14 * a) + fT / ((1. + pr) * (1 + pr)));
+      }
+      dCdrho = nsTmp * r.baseEff * invM;
+      for (size_t i = 0; i < r.tb_idx.size(); ++i) dCdrho += nsTmp * (r.tb_eff[i] * y[r.tb_idx[i]]);
+      nsTmp *= rho;
+      for (int s = 0; s < ns - 1; ++s) dCdY[s] = nsTmp * r.baseEff * (invMsp[s] - invMsp[ns - 1]);
+      for (size_t i = 0; i < r.tb_idx.size(); ++i) if (r.tb_idx[i] < ns - 1) dCdY[r.tb_idx[i]] += nsTmp * r.tb_eff[i];
+      for (size_t i = 0; i < r.tb_idx.size(); ++i)
+        if (r.tb_idx[i] == ns - 1) { for (int s = 0; s < ns - 1; ++s) dCdY[s] -= nsTmp * r.tb_eff[i]; break; }
+    }
+
+    const double q = Rnet * Ctbaf;
+    const double dqdrho = dRdrho * Ctbaf + dCdrho * Rnet;
+    const double dqdT = dRdT * Ctbaf + dCdT * Rnet;
+    for (int s = 0; s < ns - 1; ++s) dqdY[s] = dRdY[s] * Ctbaf + dCdY[s] * Rnet;
+    for (size_t i = 0; i < r.net_idx.size(); ++i) {
+      const int k = r.net_idx[i];
+      const double f = -r.net_st[i] * r.net_mw[i];
+      w[k] += f * q;
+      ws[k] += f * dqdrho;
+      ws[k + nsp1] += f * dqdT;
+      for (int s = 0; s < ns - 1; ++s) ws[k + 2 * nsp1 + nsp1 * s] += f * dqdY[s];
+    }
+  }
+}
+
+// isobaric_reactor_kernels.cpp:19-29
+void chem_rhs_isobaric(const Mech &m, double rho, double cp, const double *h, const double *w, double *rhs) {
+  double d = 0.;
+  for (int i = 0; i < m.ns; ++i) d += h[i] * w[i];
+  rhs[0] = -d / (rho * cp);
+  const double ir = 1. / rho;
+  for (int i = 0; i < m.ns - 1; ++i) rhs[1 + i] = w[i] * ir;
+}
+
+inline double dotp(int n, const double *x, const double *yv) {
+  double d = 0.;
+  for (int i = 0; i < n; ++i) d += yv[i] * x[i];
+  return d;
+}
+
+// isobaric_reactor_kernels.cpp:58-98 — primitive Jacobian ns x (ns+1) column-major
+void chem_jac_isobaric(const Mech &m, double rho, double cp, const double *cpi, double cpsensT, const double *h,
+                       const double *w, const double *ws, double *rhs, double *pj) {
+  const int n = m.ns;
+  chem_rhs_isobaric(m, rho, cp, h, w, rhs);
+  const double iRC = 1. / (rho * cp), iR = 1. / rho, iC = 1. / cp;
+  pj[0] = -iRC * dotp(n, ws, h) - iR * rhs[0];
+  for (int i = 0; i < n - 1; ++i) pj[1 + i] = iR * (ws[i] - iR * w[i]);
+  pj[n] = -iRC * (dotp(n, &ws[n + 1], h) + dotp(n, w, cpi)) - rhs[0] * cpsensT * iC;
+  for (int i = 0; i < n - 1; ++i) pj[n + 1 + i] = ws[n + 1 + i] * iR;
+  const double cpn = cpi[n - 1];
+  for (int k = 0; k < n - 1; ++k) {
+    const int fr = (2 + k) * n;
+    pj[fr] = -iRC * dotp(n, &ws[(2 + k) * (n + 1)], h) - rhs[0] * (cpi[k] - cpn) * iC;
+    for (int i = 0; i < n - 1; ++i) pj[fr + 1 + i] = iR * ws[(2 + k) * (n + 1) + i];
+  }
+}
+
+// isobaric_reactor_kernels.cpp:39-56
+void mass_rhs_isobaric(const Mech &m, const double *y, const double *h, const double *hin, double cp,
+                       const double *yin, double tau, double *rhs) {
+  const int n = m.ns;
+  rhs[0] = (hin[n - 1] - h[n - 1]) * yin[n - 1];
+  for (int i = 0; i < n - 1; ++i) { rhs[0] += (hin[i] - h[i]) * yin[i]; rhs[1 + i] = yin[i] - y[i]; }
+  rhs[0] /= cp;
+  const double it = 1. / tau;
+  for (int i = 0; i < n; ++i) rhs[i] *= it;
+}
+
+// isobaric_reactor_kernels.cpp:100-140
+void mass_jac_isobaric(const Mech &m, const double *y, double cp, double cpsensT, const double *cpi, const double *h,
+                       const double *hin, const double *yin, double tau, double *rhs, double *pj) {
+  const int n = m.ns;
+  mass_rhs_isobaric(m, y, h, hin, cp, yin, tau, rhs);
+  const double iC = 1. / cp, it = 1. / tau;
+  for (int i = 0; i < n; ++i) pj[i] = 0.;
+  pj[n] = -iC * (cpsensT * rhs[0] + it * dotp(n, yin, cpi));
+  for (int i = 0; i < n - 1; ++i) pj[n + 1 + i] = 0.;
+  for (int k = 0; k < n - 1; ++k) {
+    pj[(2 + k) * n] = -rhs[0] * (cpi[k] - cpi[n - 1]) * iC;
+    for (int i = 0; i < n - 1; ++i) pj[(2 + k) * n + 1 + i] = 0.;
+    pj[(2 + k) * n + 1 + k] = -it;
+  }
+}
+
+// isobaric_reactor_kernels.cpp:31-37
+inline double heat_rhs_isobaric(double T, double rho, double cp, double Tf, double Ts, double hc, double eps, double sov) {
+  return sov / (rho * cp) * (hc * (Tf - T) + eps * 5.67e-8 * (Ts * Ts * Ts * Ts - T * T * T * T));
+}
+
+// isobaric_reactor_kernels.cpp:319-343
+void transform_isobaric(const Mech &m, double rho, double T, double mmw, const double *pj, double *jac) {
+  const int n = m.ns;
+  for (int i = 0; i < n * n; ++i) jac[i] = 0.;
+  const double roT = rho / T;
+  for (int i = 0; i < n; ++i) jac[i] = pj[n + i] - roT * pj[i];
+  const double nrm = -rho * mmw;
+  for (int k = 0; k < n - 1; ++k)
+    for (int i = 0; i < n; ++i)
+      jac[(1 + k) * n + i] = pj[(2 + k) * n + i] + nrm * (m.invmw[k] - m.invmw[n - 1]) * pj[i];
+}
+
+// isobaric_reactor_kernels.cpp:170-219
+void reactor_rhs_isobaric(const Mech &m, const double *state, double p, double Tin, const double *yin, double tau,
+                          double Tf, double Ts, double hc, double eps, double sov, int heat, bool open, double *rhs) {
+  const int n = m.ns;
+  std::vector<double> y(n), h(n), w(n), cpi(n);
+  const double T = state[0];
+  extract_y(m, &state[1], y.data());
+  const double mmw = mmw_of(m, y.data());
+  const double rho = p * mmw / (T * m.Ru);
+  double cp;
+  cp_mix_and_species(m, T, y.data(), &cp, cpi.data());
+  species_enthalpies(m, T, h.data());
+  production_rates(m, T, rho, mmw, y.data(), w.data());
+  chem_rhs_isobaric(m, rho, cp, h.data(), w.data(), rhs);
+  if (open) {
+    std::vector<double> mr(n), hin(n);
+    species_enthalpies(m, Tin, hin.data());
+    mass_rhs_isobaric(m, y.data(), h.data(), hin.data(), cp, yin, tau, mr.data());
+    for (int i = 0; i < n; ++i) rhs[i] += mr[i];
+  }
+  if (heat == 1) rhs[0] = 0.;
+  else if (heat == 2) rhs[0] += heat_rhs_isobaric(T, rho, cp, Tf, Ts, hc, eps, sov);
+}
+
+// isobaric_reactor_kernels.cpp:221-317 (rates_sensitivity_option 0, transform option 0)
+void reactor_jac_isobaric(const Mech &m, const double *state, double p, double Tin, const double *yin, double tau,
+                          double Tf, double Ts, double hc, double eps, double sov, int heat, bool open,
+                          double *rhs, double *jac) {
+  const int n = m.ns;
+  std::vector<double> y(n), h(n), w(n), cpi(n), cpis(n), ws((n + 1) * (n + 1)), pj(n * (n + 1) + 1);
+  const double T = state[0];
+  extract_y(m, &state[1], y.data());
+  const double mmw = mmw_of(m, y.data());
+  const double rho = p * mmw / (T * m.Ru);
+  double cp, cps;
+  cp_mix_and_species(m, T, y.data(), &cp, cpi.data());
+  species_enthalpies(m, T, h.data());
+  cp_sens_T(m, T, y.data(), &cps, cpis.data());
+  prod_rates_sens_exact(m, T, rho, mmw, y.data(), w.data(), ws.data());
+  chem_jac_isobaric(m, rho, cp, cpi.data(), cps, h.data(), w.data(), ws.data(), rhs, pj.data());
+  if (open) {
+    std::vector<double> mr(n), mj(n * (n + 1) + 1), hin(n);
+    species_enthalpies(m, Tin, hin.data());
+    mass_jac_isobaric(m, y.data(), cp, cps, cpi.data(), h.data(), hin.data(), yin, tau, mr.data(), mj.data());
+    for (int i = 0; i < n * (n + 1); ++i) pj[i] += mj[i];
+    for (int i = 0; i < n; ++i) rhs[i] += mr[i];
+  }
+  if (heat == 1) {
+    for (int k = 0; k < n + 1; ++k) pj[k * n] = 0.;
+    rhs[0] = 0.;
+  } else if (heat == 2) {
+    const double q = heat_rhs_isobaric(T, rho, cp, Tf, Ts, hc, eps, sov);
+    const double iRC = 1. / (rho * cp), iC = 1. / cp;
+    // heat_jac_isobaric, isobaric_reactor_kernels.cpp:142-168
+    pj[0] += -q / rho;
+    pj[n] += -iC * cps * q - sov * iRC * (hc + 4. * eps * 5.67e-8 * T * T * T);
+    for (int k = 0; k < n - 1; ++k) pj[(2 + k) * n] += iC * q * (cpi[n - 1] - cpi[k]);
+    rhs[0] += q;
+  }
+  transform_isobaric(m, rho, T, mmw, pj.data(), jac);
+}
+
+// per-point isobaric chemistry block used by the flamelet Jacobian (flamelet_kernels.cpp:1249-1327)
+void flamelet_point_block(const Mech &m, const double *st, double p, bool adiabatic, double Tc, double Tr, double hcv,
+                          double hrd, bool scaled, double maxT, double maxT4, double *cp_out, double *cps_out,
+                          double *block) {
+  const int n = m.ns;
+  std::vector<double> y(n), h(n), w(n), cpi(n), cpis(n), ws((n + 1) * (n + 1)), pj(n * (n + 1) + 1), rt(n);
+  const double T = st[0];
+  extract_y(m, &st[1], y.data());
+  const double mmw = mmw_of(m, y.data());
+  const double rho = p * mmw / (T * m.Ru);
+  double cp, cps;
+  cp_mix_and_species(m, T, y.data(), &cp, cpi.data());
+  species_enthalpies(m, T, h.data());
+  cp_sens_T(m, T, y.data(), &cps, cpis.data());
+  prod_rates_sens_exact(m, T, rho, mmw, y.data(), w.data(), ws.data());
+  chem_jac_isobaric(m, rho, cp, cpi.data(), cps, h.data(), w.data(), ws.data(), rt.data(), pj.data());
+  if (!adiabatic) {
+    const double iRC = 1. / (rho * cp), iC = 1. / cp;
+    double q;
+    if (scaled) {
+      const double Tr4 = Tr * Tr * Tr * Tr;
+      q = (hcv * (Tc - T) / (maxT - Tc) + hrd * 5.67e-8 * (Tr4 - T * T * T * T) / (maxT4 - Tr4)) * iRC;
+      pj[n] -= iC * cps * q + iRC * (hcv / (maxT - Tc) + 4. * hrd / (maxT4 - Tr4) * 5.67e-8 * T * T * T);
+    } else {
+      q = (hcv * (Tc - T) + hrd * 5.67e-8 * (Tr * Tr * Tr * Tr - T * T * T * T)) * iRC;
+      pj[n] -= iC * cps * q + iRC * (hcv + 4. * hrd * 5.67e-8 * T * T * T);
+    }
+    pj[0] -= q / rho;
+    // flamelet_kernels.cpp:1316-1319 loops k < ns+1 and reads cpi[ns-1], cpi[ns] out of bounds for the
+    // last two columns; the in-bounds columns (k < ns-1) are restated here.
+    const double cpn = cpi[n - 1];
+    for (int k = 0; k < n - 1; ++k) pj[(2 + k) * n] += iC * q * (cpn - cpi[k]);
+  }
+  transform_isobaric(m, rho, T, mmw, pj.data(), block);
+  *cp_out = cp;
+  *cps_out = cps;
+}
+
+double cp_mix_only(const Mech &m, double T, const double *y) {
+  std::vector<double> cpi(m.ns);
+  double cp;
+  cp_mix_and_species(m, T, y, &cp, cpi.data());
+  return cp;
+}
+
+// --- dense LU with partial pivoting (LAPACK dgetrf/dgetrs semantics: 1-based pivots, first max |a|) ---
+void lu_factor(int n, double *a, int *piv) {
+  for (int j = 0; j < n; ++j) {
+    int p = j;
+    double amax = std::abs(a[j + j * n]);
+    for (int i = j + 1; i < n; ++i) if (std::abs(a[i + j * n]) > amax) { amax = std::abs(a[i + j * n]); p = i; }
+    piv[j] = p + 1;
+    if (p != j) for (int c = 0; c < n; ++c) std::swap(a[j + c * n], a[p + c * n]);
+    const double d = a[j + j * n];
+    if (d != 0.) {
+      const double id = 1. / d;
+      for (int i = j + 1; i < n; ++i) a[i + j * n] *= id;
+    }
+    for (int c = j + 1; c < n; ++c) {
+      const double f = a[j + c * n];
+      if (f != 0.) for (int i = j + 1; i < n; ++i) a[i + c * n] -= a[i + j * n] * f;
+    }
+  }
+}
+
+void lu_solve(int n, const double *a, const int *piv, double *b) {
+  for (int i = 0; i < n; ++i) { const int p = piv[i] - 1; if (p != i) std::swap(b[i], b[p]); }
+  for (int j = 0; j < n; ++j) { const double bj = b[j]; if (bj != 0.) for (int i = j + 1; i < n; ++i) b[i] -= a[i + j * n] * bj; }
+  for (int j = n - 1; j >= 0; --j) { b[j] /= a[j + j * n]; const double bj = b[j]; if (bj != 0.) for (int i = 0; i < j; ++i) b[i] -= a[i + j * n] * bj; }
+}
+
+}  // namespace
+
+// =====================================================================================================
+// C API (ctypes); every function returns 0 on success, nonzero on error.
+// =====================================================================================================
+extern "C" {
+
+void *gfo_mech_create(int ns, const double *mw, double Ru, double pref, double Tref) {
+  Mech *m = new Mech();
+  m->ns = ns;
+  m->Ru = Ru;
+  m->pref = pref;
+  m->Tref = Tref;
+  m->mw.assign(mw, mw + ns);
+  m->invmw.resize(ns);
+  for (int i = 0; i < ns; ++i) m->invmw[i] = 1. / mw[i];
+  m->cpkind.assign(ns, CP_UNKNOWN);
+  m->Tmin.assign(ns, 0.);
+  m->Tmax.assign(ns, 0.);
+  m->c.assign(ns, std::vector<double>(15, 0.));
+  return m;
+}
+
+void gfo_mech_destroy(void *h) { delete static_cast<Mech *>(h); }
+
+// chemistry_setup.cpp:102-130
+int gfo_mech_set_nasa7(void *h, int i, double Tmin, double Tmid, double Tmax, const double *low, const double *high) {
+  Mech &m = *static_cast<Mech *>(h);
+  if (i < 0 || i >= m.ns) return 1;
+  m.cpkind[i] = CP_NASA7;
+  m.Tmin[i] = Tmin;
+  m.Tmax[i] = Tmax;
+  double *c = m.c[i].data();
+  c[0] = Tmid;
+  for (int k = 0; k < 7; ++k) c[1 + k] = high[k] * m.Ru;
+  for (int k = 0; k < 7; ++k) c[8 + k] = low[k] * m.Ru;
+  c[2] /= 2.; c[3] /= 6.; c[4] /= 12.; c[5] /= 20.;
+  c[9] /= 2.; c[10] /= 6.; c[11] /= 12.; c[12] /= 20.;
+  return 0;
+}
+
+// chemistry_setup.cpp:89-100
+int gfo_mech_set_const_cp(void *h, int i, double Tmin, double Tmax, double T0, double h0, double s0, double cp) {
+  Mech &m = *static_cast<Mech *>(h);
+  if (i < 0 || i >= m.ns) return 1;
+  m.cpkind[i] = CP_CONST;
+  m.Tmin[i] = Tmin;
+  m.Tmax[i] = Tmax;
+  m.c[i][0] = T0; m.c[i][1] = h0; m.c[i][2] = s0; m.c[i][3] = cp;
+  return 0;
+}
+
+// chemistry_setup.cpp:156-345 (all mechanism_add_reaction_* variants); type 1..4
+int gfo_mech_add_reaction(void *h, int type, int reversible, int n_r, const int *r_idx, const int *r_st, int n_p,
+                          const int *p_idx, const int *p_st, const double *kfwd, int n_tb, const int *tb_idx,
+                          const double *tb_eff, double default_eff, const double *kpres, const double *troe,
+                          int n_ord, const int *ord_idx, const double *ord_val) {
+  Mech &m = *static_cast<Mech *>(h);
+  try {
+    Rxn r;
+    r.type = type;
+    r.reversible = reversible != 0;
+    for (int i = 0; i < n_r; ++i) if (r_st[i] < 1 || r_st[i] > 3) return 3;
+    for (int i = 0; i < n_p; ++i) if (p_st[i] < 1 || p_st[i] > 3) return 3;
+    for (int i = 0; i < n_r; ++i) { r.rc_idx.push_back(r_idx[i]); r.rc_st.push_back(r_st[i]); r.rc_invmw.push_back(m.invmw[r_idx[i]]); }
+    for (int i = 0; i < n_p; ++i) { r.pd_idx.push_back(p_idx[i]); r.pd_st.push_back(-p_st[i]); r.pd_invmw.push_back(m.invmw[p_idx[i]]); }
+    for (int i = 0; i < 3; ++i) r.kf[i] = kfwd[i];
+    if (type != RK_SIMPLE) {
+      r.baseEff = default_eff;
+      for (int i = 0; i < n_tb; ++i) { r.tb_idx.push_back(tb_idx[i]); r.tb_eff.push_back(m.invmw[tb_idx[i]] * (tb_eff[i] - default_eff)); }
+    }
+    if (type == RK_LINDEMANN || type == RK_TROE) for (int i = 0; i < 3; ++i) r.kp[i] = kpres[i];
+    if (type == RK_TROE) for (int i = 0; i < 4; ++i) r.troe[i] = troe[i];
+    if (n_ord > 0) {
+      r.hasOrders = true;
+      for (int i = 0; i < n_ord; ++i) {
+        r.sp_idx.push_back(ord_idx[i]);
+        r.sp_invmw.push_back(m.invmw[ord_idx[i]]);
+        r.sp_ord.push_back(ord_val[i]);
+        r.sp_nz.push_back(std::abs(ord_val[i]) > 1.e-12);
+      }
+    }
+    finalize(r, m);
+    m.rx.push_back(r);
+  } catch (...) {
+    return 2;
+  }
+  return 0;
+}
+
+int gfo_n_reactions(void *h) { return (int)static_cast<Mech *>(h)->rx.size(); }
+
+int gfo_production_rates(void *h, double T, double rho, const double *y, double *w) {
+  const Mech &m = *static_cast<Mech *>(h);
+  try { production_rates(m, T, rho, mmw_of(m, y), y, w); } catch (...) { return 2; }
+  return 0;
+}
+
+int gfo_prod_rates_sens(void *h, double rho, double T, const double *y, double *w, double *ws) {
+  const Mech &m = *static_cast<Mech *>(h);
+  try { prod_rates_sens_exact(m, T, rho, mmw_of(m, y), y, w, ws); } catch (...) { return 2; }
+  return 0;
+}
+
+int gfo_thermo(void *h, double T, const double *y, double *out_scalars, double *cpi, double *hi, double *cpis) {
+  // out_scalars: mmw, cp_mix, cp_sens_T, enthalpy_mix
+  const Mech &m = *static_cast<Mech *>(h);
+  double cp, cps;
+  cp_mix_and_species(m, T, y, &cp, cpi);
+  cp_sens_T(m, T, y, &cps, cpis);
+  species_enthalpies(m, T, hi);
+  out_scalars[0] = mmw_of(m, y);
+  out_scalars[1] = cp;
+  out_scalars[2] = cps;
+  out_scalars[3] = dotp(m.ns, hi, y);
+  return 0;
+}
+
+int gfo_reactor_rhs_isobaric(void *h, const double *state, double p, double Tin, const double *yin, double tau,
+                             double Tf, double Ts, double hc, double eps, double sov, int heat, int open, double *rhs) {
+  try { reactor_rhs_isobaric(*static_cast<Mech *>(h), state, p, Tin, yin, tau, Tf, Ts, hc, eps, sov, heat, open != 0, rhs); }
+  catch (...) { return 2; }
+  return 0;
+}
+
+int gfo_reactor_jac_isobaric(void *h, const double *state, double p, double Tin, const double *yin, double tau,
+                             double Tf, double Ts, double hc, double eps, double sov, int heat, int open,
+                             double *rhs, double *jac) {
+  try { reactor_jac_isobaric(*static_cast<Mech *>(h), state, p, Tin, yin, tau, Tf, Ts, hc, eps, sov, heat, open != 0, rhs, jac); }
+  catch (...) { return 2; }
+  return 0;
+}
+
+// Batched closed adiabatic isobaric RHS + Jacobian over N states, state SoA x[v*N + i], p[i];
+// out rhs[i*ns + r], jac[i*ns*ns + c*ns + r]. OpenMP over states when nthreads > 0 (the CPU baseline).
+int gfo_batch_rhs_jac(void *h, int N, const double *stateSoA, const double *p, double *rhs, double *jac, int nthreads) {
+  const Mech &m = *static_cast<Mech *>(h);
+  const int n = m.ns;
+  int err = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16) if (nthreads != 1)
+#endif
+  for (int i = 0; i < N; ++i) {
+    std::vector<double> st(n);
+    for (int v = 0; v < n; ++v) st[v] = stateSoA[(size_t)v * N + i];
+    try {
+      reactor_jac_isobaric(m, st.data(), p[i], 0., nullptr, 0., 0., 0., 0., 0., 0., 0, false,
+                           &rhs[(size_t)i * n], &jac[(size_t)i * n * n]);
+    } catch (...) { err = 2; }
+  }
+  return err;
+}
+
+int gfo_batch_rhs(void *h, int N, const double *stateSoA, const double *p, double *rhs, int nthreads) {
+  const Mech &m = *static_cast<Mech *>(h);
+  const int n = m.ns;
+  int err = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16) if (nthreads != 1)
+#endif
+  for (int i = 0; i < N; ++i) {
+    std::vector<double> st(n);
+    for (int v = 0; v < n; ++v) st[v] = stateSoA[(size_t)v * N + i];
+    try {
+      reactor_rhs_isobaric(m, st.data(), p[i], 0., nullptr, 0., 0., 0., 0., 0., 0., 0, false, &rhs[(size_t)i * n]);
+    } catch (...) { err = 2; }
+  }
+  return err;
+}
+
+// flamelet_kernels.cpp:31-48
+int gfo_flamelet_stencils(void *h, const double *dz, int nzi, const double *chi, const double *invLe, double *cmaj,
+                          double *csub, double *csup, double *mco, double *nco) {
+  const Mech &m = *static_cast<Mech *>(h);
+  const int n = m.ns;
+  for (int i = 0; i < nzi; ++i) {
+    const double dzt = dz[i] + dz[i + 1];
+    for (int l = 0; l < n; ++l) {
+      cmaj[i * n + l] = -chi[1 + i] / (dz[i] * dz[i + 1]) * invLe[l];
+      csub[i * n + l] = chi[1 + i] / (dzt * dz[i]) * invLe[l];
+      csup[i * n + l] = chi[1 + i] / (dzt * dz[i + 1]) * invLe[l];
+    }
+    nco[i] = 1 / (dz[i] + dz[i + 1]);
+    mco[i] = -nco[i];
+  }
+  return 0;
+}
+
+// flamelet_kernels.cpp:50-90
+int gfo_flamelet_jac_indices(void *h, int nzi, int *rows, int *cols) {
+  const int n = static_cast<Mech *>(h)->ns;
+  int idx = 0;
+  for (int iz = 0; iz < nzi; ++iz)
+    for (int iq = 0; iq < n; ++iq)
+      for (int jq = 0; jq < n; ++jq) { rows[idx] = iz * n + jq; cols[idx] = iz * n + iq; ++idx; }
+  for (int iz = 1; iz < nzi; ++iz)
+    for (int iq = 0; iq < n; ++iq) { rows[idx] = iz * n + iq; cols[idx] = iz * n + iq - n; ++idx; }
+  for (int iz = 0; iz < nzi - 1; ++iz)
+    for (int iq = 0; iq < n; ++iq) { rows[idx] = iz * n + iq; cols[idx] = iz * n + iq + n; ++idx; }
+  return 0;
+}
+
+// flamelet_kernels.cpp:1039-1218 (C++ argument order: T_conv, h_conv, T_rad, h_rad)
+int gfo_flamelet_rhs(void *h, const double *state, double p, const double *oxy, const double *fuel, int adiabatic,
+                     const double *Tconv, const double *hconv, const double *Trad, const double *hrad, int nzi,
+                     const double *cmaj, const double *csub, const double *csup, const double *mco, const double *nco,
+                     const double *chi, int incl_ef, int incl_vcp, int scaled_hl, double *out) {
+  const Mech &m = *static_cast<Mech *>(h);
+  const int n = m.ns;
+  try {
+    double maxT = -1, maxT4 = -1;
+    if (scaled_hl) {
+      for (int i = 0; i < nzi; ++i) maxT = std::max(maxT, state[i * n]);
+      maxT4 = maxT * maxT * maxT * maxT;
+    }
+    std::vector<double> cpz(nzi), y(n), h(n), w(n), cpi(n);
+    if (incl_vcp) {
+      std::vector<double> cpg(nzi);
+      for (int i = 0; i < nzi; ++i) { extract_y(m, &state[i * n + 1], y.data()); cpg[i] = cp_mix_only(m, state[i * n], y.data()); }
+      for (int i = 0; i < nzi; ++i) {
+        if (i == 0) { extract_y(m, &oxy[1], y.data()); cpz[i] = mco[i] * cp_mix_only(m, oxy[0], y.data()) + nco[i] * cpg[1]; }
+        else if (i == nzi - 1) { extract_y(m, &fuel[1], y.data()); cpz[i] = mco[i] * cpg[nzi - 2] + nco[i] * cp_mix_only(m, fuel[0], y.data()); }
+        else cpz[i] = mco[i] * cpg[i - 1] + nco[i] * cpg[i + 1];
+      }
+    }
+    for (int i = 0; i < nzi; ++i) {
+      const double T = state[i * n];
+      extract_y(m, &state[i * n + 1], y.data());
+      const double mmw = mmw_of(m, y.data());
+      const double rho = p * mmw / (T * m.Ru);
+      double cp;
+      cp_mix_and_species(m, T, y.data(), &cp, cpi.data());
+      species_enthalpies(m, T, h.data());
+      production_rates(m, T, rho, mmw, y.data(), w.data());
+      chem_rhs_isobaric(m, rho, cp, h.data(), w.data(), &out[i * n]);
+      if (!adiabatic) {
+        const double hc = hconv[i], hr = hrad[i], Tc = Tconv[i], Tr = Trad[i];
+        double q;
+        if (scaled_hl) {
+          const double Tr4 = Tr * Tr * Tr * Tr;
+          q = hc * (Tc - T) / (maxT - Tc) + hr * 5.67e-8 * (Tr4 - T * T * T * T) / (maxT4 - Tr4);
+        } else {
+          q = hc * (Tc - T) + hr * 5.67e-8 * (Tr * Tr * Tr * Tr - T * T * T * T);
+        }
+        out[i * n] += q / (rho * cp);
+      }
+      const double *sm = (i == 0) ? oxy : &state[(i - 1) * n];
+      const double *sp = (i == nzi - 1) ? fuel : &state[(i + 1) * n];
+      if (incl_ef) {
+        const double cpn = cpi[n - 1];
+        const double dTdZ = mco[i] * sm[0] + nco[i] * sp[0];
+        double dY = 0.;
+        for (int j = 0; j < n - 1; ++j) dY += (cpi[j] - cpn) * (mco[i] * sm[1 + j] + nco[i] * sp[1 + j]);
+        out[i * n] += 0.5 * chi[i] / cp * dTdZ * dY;
+        if (incl_vcp) out[i * n] += 0.5 * chi[i] * cpz[i] / cp * dTdZ;
+      }
+      if (incl_vcp && !incl_ef) {
+        const double dTdZ = mco[i] * sm[0] + nco[i] * sp[0];
+        out[i * n] += 0.5 * chi[i] * cpz[i] / cp * dTdZ;
+      }
+    }
+    const int end = (nzi - 1) * n;
+    for (int i = n; i < end; ++i) out[i] += cmaj[i] * state[i] + csub[i] * state[i - n] + csup[i] * state[i + n];
+    for (int j = 0; j < n; ++j) {
+      out[j] += cmaj[j] * state[j] + csub[j] * oxy[j] + csup[j] * state[j + n];
+      out[end + j] += cmaj[end + j] * state[end + j] + csub[end + j] * state[end + j - n] + csup[end + j] * fuel[j];
+    }
+  } catch (...) { return 2; }
+  return 0;
+}
+
+// flamelet_kernels.cpp:1220-1409. Eigenvalues are not computed here: out_blocks_pre (if non-null)
+// receives the per-point chemistry blocks before the diffusion diagonal is added, from which the
+// caller computes the eigenvalues exactly as the reference does (LAPACK dgeev on that block).
+int gfo_flamelet_jacobian(void *h, const double *state, double p, const double *oxy, const double *fuel, int adiabatic,
+                          const double *Tconv, const double *hconv, const double *Trad, const double *hrad, int nzi,
+                          const double *cmaj, const double *csub, const double *csup, const double *mco,
+                          const double *nco, const double *chi, int scale_and_offset, double prefactor, int incl_ef,
+                          int incl_vcp, int scaled_hl, double *out_blocks_pre, double *out) {
+  const Mech &m = *static_cast<Mech *>(h);
+  const int n = m.ns;
+  const int bs = n * n;
+  const long nel = (long)n * (nzi * n + 2 * (nzi - 1));
+  try {
+    double maxT = -1, maxT4 = -1;
+    if (scaled_hl) {
+      for (int i = 0; i < nzi; ++i) maxT = std::max(maxT, state[i * n]);
+      maxT4 = maxT * maxT * maxT * maxT;
+    }
+    std::vector<double> cp(nzi), cps(nzi), y(n);
+    for (int iz = 0; iz < nzi; ++iz) {
+      double *blk = &out[(long)iz * bs];
+      flamelet_point_block(m, &state[iz * n], p, adiabatic != 0, adiabatic ? 0. : Tconv[iz], adiabatic ? 0. : Trad[iz],
+                           adiabatic ? 0. : hconv[iz], adiabatic ? 0. : hrad[iz], scaled_hl != 0, maxT, maxT4,
+                           &cp[iz], &cps[iz], blk);
+      if (out_blocks_pre) std::memcpy(&out_blocks_pre[(long)iz * bs], blk, sizeof(double) * bs);
+      for (int iq = 0; iq < n; ++iq) blk[iq * (n + 1)] += cmaj[iz * n + iq];
+    }
+    if (incl_ef) {
+      for (int i = 1; i < nzi - 1; ++i) {
+        const double dTdZ = mco[i] * state[(i - 1) * n] + nco[i] * state[(i + 1) * n];
+        const double dcpdZ = mco[i] * cp[i - 1] + nco[i] * cp[i + 1];
+        const double f1 = 0.5 * chi[i] / cp[i] * dTdZ * dcpdZ;
+        out[(long)i * bs] -= f1 / cp[i] * cps[i];
+      }
+      {
+        extract_y(m, &oxy[1], y.data());
+        const double cpo = cp_mix_only(m, oxy[0], y.data());
+        const int i = 0;
+        const double dTdZ = mco[i] * oxy[0] + nco[i] * state[(i + 1) * n];
+        const double dcpdZ = mco[i] * cpo + nco[i] * cp[i + 1];
+        const double f1 = 0.5 * chi[i] / cp[i] * dTdZ * dcpdZ;
+        out[(long)i * bs] -= f1 / cp[i] * cps[i];
+      }
+      {
+        extract_y(m, &fuel[1], y.data());
+        const double cpf = cp_mix_only(m, fuel[0], y.data());
+        const int i = nzi - 1;
+        const double dTdZ = mco[i] * state[(i - 1) * n] + nco[i] * fuel[0];
+        const double dcpdZ = mco[i] * cp[i - 1] + nco[i] * cpf;
+        const double f1 = 0.5 * chi[i] / cp[i] * dTdZ * dcpdZ;
+        out[(long)i * bs] -= f1 / cp[i] * cps[i];
+      }
+    }
+    long idx = (long)nzi * bs;
+    const long off = (long)(nzi - 1) * n;
+    for (int iz = 1; iz < nzi; ++iz)
+      for (int iq = 0; iq < n; ++iq) {
+        out[idx] += csub[iz * n + iq];
+        out[off + idx] += csup[(iz - 1) * n + iq];
+        ++idx;
+      }
+    if (scale_and_offset) {
+      for (long i = 0; i < nel; ++i) out[i] *= prefactor;
+      for (int iz = 0; iz < nzi; ++iz)
+        for (int iq = 0; iq < n; ++iq) out[(long)iz * bs + iq * (n + 1)] -= 1.;
+    }
+  } catch (...) { return 2; }
+  return 0;
+}
+
+// btddod_matrix_kernels.cpp:19-80 — block Thomas factorization, in place
+int gfo_btddod_full_factorize(double *d, int nb, int bs, double *l, int *piv) {
+  const int nb2 = bs * bs;
+  const long noff = (long)(nb - 1) * bs;
+  const double *sub = &d[(long)nb * nb2];
+  const double *sup = &d[(long)nb * nb2 + noff];
+  lu_factor(bs, d, piv);
+  for (int i = 1; i < nb; ++i) {
+    const long o1 = (long)i * nb2, om = o1 - nb2;
+    double *L = &l[o1];
+    for (int e = 0; e < nb2; ++e) L[e] = 0.;
+    for (int c = 0; c < bs; ++c) L[c * (bs + 1)] = 1.;
+    for (int c = 0; c < bs; ++c) lu_solve(bs, &d[om], &piv[(i - 1) * bs], &L[c * bs]);
+    const int b1 = (i - 1) * bs;
+    for (int j = 0; j < bs; ++j) for (int k = 0; k < bs; ++k) L[j * bs + k] *= sub[b1 + k];
+    for (int j = 0; j < bs; ++j) {
+      const double f = -sup[b1 + j];
+      for (int k = 0; k < bs; ++k) d[o1 + j * bs + k] += f * L[j * bs + k];
+    }
+    lu_factor(bs, &d[o1], &piv[i * bs]);
+  }
+  return 0;
+}
+
+// btddod_matrix_kernels.cpp:82-119
+int gfo_btddod_full_solve(const double *d, const double *l, const int *piv, const double *rhs, int nb, int bs, double *x) {
+  const int nb2 = bs * bs;
+  const long noff = (long)(nb - 1) * bs;
+  const double *sup = &d[(long)nb * nb2 + noff];
+  std::vector<double> y((long)nb * bs), t(bs);
+  for (int i = 0; i < bs; ++i) y[i] = rhs[i];
+  for (int i = 1; i < nb; ++i) {
+    for (int k = 0; k < bs; ++k) y[i * bs + k] = rhs[i * bs + k];
+    const double *L = &l[(long)i * nb2];
+    for (int c = 0; c < bs; ++c) {
+      const double a = -1. * y[(i - 1) * bs + c];
+      for (int k = 0; k < bs; ++k) y[i * bs + k] = y[i * bs + k] + L[c * bs + k] * a;
+    }
+  }
+  const int last = nb - 1;
+  for (int k = 0; k < bs; ++k) x[last * bs + k] = y[last * bs + k];
+  lu_solve(bs, &d[(long)last * nb2], &piv[last * bs], &x[last * bs]);
+  for (int i = nb - 2; i >= 0; --i) {
+    for (int k = 0; k < bs; ++k) t[k] = y[i * bs + k] - sup[i * bs + k] * x[(i + 1) * bs + k];
+    lu_solve(bs, &d[(long)i * nb2], &piv[i * bs], t.data());
+    for (int k = 0; k < bs; ++k) x[i * bs + k] = t[k];
+  }
+  return 0;
+}
+
+// btddod_matrix_kernels.cpp:448-465
+int gfo_btddod_scale_and_add_diagonal(double *v, double a, const double *dg, double b, int nb, int bs) {
+  const long nel = (long)bs * (nb * bs + 2 * (nb - 1));
+  for (long i = 0; i < nel; ++i) v[i] *= a;
+  for (int iz = 0; iz < nb; ++iz)
+    for (int iq = 0; iq < bs; ++iq) v[(long)iz * bs * bs + iq * (bs + 1)] += b * dg[iz * bs + iq];
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+/*
+ * griffon_amd.h — C ABI of the MI355X (gfx950) implementation of Spitfire's Griffon chemistry hot path.
+ *
+ * Two kinds of entry points:
+ *   1. Drop-ins for the reference's Cython boundary (src/spitfire/griffon/griffon.pyx): same argument
+ *      order and meaning, host pointers, synchronous; the arithmetic runs on the GPU (the library
+ *      stages the arrays through device memory). Each cites the griffon.pyx line it replaces.
+ *   2. Batched device entry points: device pointers, many independent states / grid points /
+ *      flamelets per call, ordered on the caller's HIP stream (passed as void*).
+ *
+ * Conventions: every function returns GF_OK (0) or a nonzero GF_ERR_* code; gf_last_error() gives the
+ * message of the calling thread's last failure. No exceptions cross this boundary. All floating-point
+ * data is FP64 unless a name says f32. Jacobians are column-major ns x ns blocks (row i, column j at
+ * i + j*ns), as in griffon. State vectors are [T, Y_0 .. Y_{ns-2}].
+ */
+#ifndef GRIFFON_AMD_H
+#define GRIFFON_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GF_OK 0
+#define GF_ERR_ARG 1          /* bad argument (index, size, null pointer) */
+#define GF_ERR_STATE 2        /* call not valid in the mechanism's current state */
+#define GF_ERR_UNSUPPORTED 3  /* feature outside this implementation's scope */
+#define GF_ERR_HIP 4          /* HIP runtime failure (no device, launch error, ...) */
+
+typedef struct gf_mech gf_mech;
+
+const char *gf_last_error(void);
+int gf_device_count(void);
+
+/* ---------------- mechanism construction (griffon.pyx:234-551, chemistry_setup.cpp:21-732) ----------- */
+/* Species are added in order; name->index resolution and the element map stay with the caller, which
+ * passes each species' molecular weight (the reference sums element weights in mechanism_add_species,
+ * chemistry_setup.cpp:36-62). Reactant/product/efficiency/order lists are index arrays given in the
+ * order griffon's std::map would iterate them (sorted species names). */
+int gf_mech_create(gf_mech **out);
+int gf_mech_destroy(gf_mech *m);
+int gf_mechanism_set_gas_constant(gf_mech *m, double Ru);              /* griffon.pyx mechanism_set_gas_constant */
+int gf_mechanism_set_ref_pressure(gf_mech *m, double p_ref);           /* griffon.pyx mechanism_set_ref_pressure */
+int gf_mechanism_set_ref_temperature(gf_mech *m, double T_ref);        /* griffon.pyx mechanism_set_ref_temperature */
+int gf_mechanism_add_species(gf_mech *m, double molecular_weight);     /* griffon.pyx mechanism_add_species */
+int gf_mechanism_resize_heat_capacity_data(gf_mech *m);                /* griffon.pyx mechanism_resize_heat_capacity_data */
+int gf_mechanism_add_const_cp(gf_mech *m, int species, double Tmin, double Tmax, double T0, double h0, double s0,
+                              double cp);                              /* griffon.pyx mechanism_add_const_cp */
+int gf_mechanism_add_nasa7_cp(gf_mech *m, int species, double Tmin, double Tmid, double Tmax,
+                              const double *low_coeffs7, const double *high_coeffs7); /* mechanism_add_nasa7_cp */
+/* type: 1 simple, 2 three-body, 3 Lindemann, 4 Troe (all mechanism_add_reaction_* variants of
+ * griffon.pyx, with or without special orders); kfwd = {A, b, Ea/R}, kpres likewise (falloff low-pressure
+ * rate), troe = {alpha, T***, T*, T**}. */
+int gf_mechanism_add_reaction(gf_mech *m, int type, int reversible, int n_reactants, const int *reactant_idx,
+                              const int *reactant_stoich, int n_products, const int *product_idx,
+                              const int *product_stoich, const double *kfwd, int n_third_body, const int *tb_idx,
+                              const double *tb_efficiency, double default_efficiency, const double *kpres,
+                              const double *troe, int n_orders, const int *order_idx, const double *order_value);
+int gf_mech_n_species(const gf_mech *m);
+int gf_mech_n_reactions(const gf_mech *m);
+
+/* ---------------- drop-ins for PyCombustionKernels compute methods (host pointers) ------------------- */
+int gf_production_rates(gf_mech *m, double T, double rho, const double *y, double *out_w);         /* griffon.pyx:763-768 */
+int gf_prod_rates_primitive_sensitivities(gf_mech *m, double rho, double T, const double *y, int rates_sensitivity_option,
+                                          double *out_sens);                                        /* griffon.pyx:773-783 */
+int gf_reactor_rhs_isobaric(gf_mech *m, const double *state, double pressure, double inflow_T, const double *inflow_y,
+                            double tau, double fluid_T, double surf_T, double h_conv, double eps_rad,
+                            double surface_area_over_volume, int heat_transfer_option, int open,
+                            double *out_rhs);                                                       /* griffon.pyx:788-802 */
+int gf_reactor_jac_isobaric(gf_mech *m, const double *state, double pressure, double inflow_T, const double *inflow_y,
+                            double tau, double fluid_T, double surf_T, double h_conv, double eps_rad,
+                            double surface_area_over_volume, int heat_transfer_option, int open,
+                            int rates_sensitivity_option, int sensitivity_transform_option, double *out_rhs,
+                            double *out_jac);                                                       /* griffon.pyx:807-824 */
+int gf_flamelet_stencils(gf_mech *m, const double *dz, int nzi, const double *dissipation_rate,
+                         const double *inv_lewis, double *out_cmajor, double *out_csub, double *out_csup,
+                         double *out_mcoeff, double *out_ncoeff);                                   /* griffon.pyx:556-566 */
+int gf_flamelet_jac_indices(gf_mech *m, int nzi, int *out_rows, int *out_cols);                     /* griffon.pyx:571-575 */
+/* heat-loss arrays in the C++ order T_conv, h_conv, T_rad, h_rad (griffon.pyx:606-609 reorders the Python
+ * order T_conv, T_rad, h_conv, h_rad into this one). */
+int gf_flamelet_rhs(gf_mech *m, const double *state, double pressure, const double *oxy_state, const double *fuel_state,
+                    int adiabatic, const double *T_conv, const double *h_conv, const double *T_rad,
+                    const double *h_rad, int nzi, const double *cmajor, const double *csub, const double *csup,
+                    const double *mcoeff, const double *ncoeff, const double *chi, int include_enthalpy_flux,
+                    int include_variable_cp, int use_scaled_heat_loss, double *out_rhs);           /* griffon.pyx:580-620 */
+int gf_flamelet_jacobian(gf_mech *m, const double *state, double pressure, const double *oxy_state,
+                         const double *fuel_state, int adiabatic, const double *T_conv, const double *h_conv,
+                         const double *T_rad, const double *h_rad, int nzi, const double *cmajor,
+                         const double *csub, const double *csup, const double *mcoeff, const double *ncoeff,
+                         const double *chi, int compute_eigenvalues, double diffterm, int scale_and_offset,
+                         double prefactor, int rates_sensitivity_option, int sensitivity_transform_option,
+                         int include_enthalpy_flux, int include_variable_cp, int use_scaled_heat_loss,
+                         double *out_expeig, double *out_jac);                                     /* griffon.pyx:625-679 */
+
+/* ---------------- drop-ins for py_btddod_* (griffon.pyx:1074-1113) ------------------------------------ */
+int gf_btddod_full_factorize(double *inout_values, int num_blocks, int block_size, double *out_l_values,
+                             int *out_pivots);                                                      /* griffon.pyx:1074-1079 */
+int gf_btddod_full_solve(const double *d_factors, const double *l_values, const int *pivots, const double *rhs,
+                         int num_blocks, int block_size, double *out_solution);                    /* griffon.pyx:1084-1091 */
+int gf_btddod_scale_and_add_diagonal(double *inout_values, double matrix_scale, const double *diagonal,
+                                     double diag_scale, int num_blocks, int block_size);          /* griffon.pyx:1107-1113 */
+
+/* ---------------- batched device entry points (device pointers, stream-ordered) ---------------------- */
+/* Closed adiabatic isobaric reactor chemistry over n independent states. Element v of state i is read
+ * from x[i*x_item_stride + v*x_var_stride] (x_item_stride=1, x_var_stride=n is structure-of-arrays);
+ * p[i*p_stride] is the pressure (p_stride 0 = one pressure). rhs element v of state i goes to
+ * rhs[i*rhs_item_stride + v*rhs_var_stride]; the ns*ns Jacobian of state i to jac + i*jac_item_stride.
+ * Replaces n calls of reactor_rhs_isobaric / reactor_jac_isobaric (griffon.pyx:788-824) with
+ * heat_transfer_option 0, open false, rates_sensitivity_option 0, sensitivity_transform_option 0. */
+int gf_batch_reactor_rhs_isobaric(gf_mech *m, int n, const double *x, long x_item_stride, long x_var_stride,
+                                  const double *p, long p_stride, double *rhs, long rhs_item_stride,
+                                  long rhs_var_stride, void *stream);
+int gf_batch_reactor_jac_isobaric(gf_mech *m, int n, const double *x, long x_item_stride, long x_var_stride,
+                                  const double *p, long p_stride, double *rhs, long rhs_item_stride,
+                                  long rhs_var_stride, double *jac, long jac_item_stride, void *stream);
+
+/* Batched block-tridiagonal (BTDDOD) factor/solve over nsys independent systems of possibly different
+ * block counts: system s has nblocks[s] blocks of size bs; its values start at values + voff[s] (griffon
+ * layout: nblocks*bs*bs block entries, then (nblocks-1)*bs sub, then (nblocks-1)*bs sup), its L blocks at
+ * l_values + loff[s], pivots at pivots + poff[s], vectors at offset xoff[s]. FP64 throughout. */
+int gf_batch_btddod_full_factorize(int nsys, const int *nblocks, int bs, double *values, const long *voff,
+                                   double *l_values, const long *loff, int *pivots, const long *poff, void *stream);
+int gf_batch_btddod_full_solve(int nsys, const int *nblocks, int bs, const double *values, const long *voff,
+                               const double *l_values, const long *loff, const int *pivots, const long *poff,
+                               const double *rhs, double *x, const long *xoff, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -282,16 +282,19 @@ inline double troe_fcent(const Rxn &r, double T, double invT) {
   }
 }
 
+// Mass-action product k * prod C_i^nu_i. `grouped` reproduces production_rates' `k *= C0 * C1` association
+// (chemistry_kernels.cpp:373-451); the sensitivity code multiplies left to right, `kf * C0 * C1`
+// (rates_sensitivities_exact.cpp:287-325, 561-599).
 inline double mass_action(const std::vector<int> &idx, const std::vector<int> &st, const std::vector<double> &invmw,
-                          int order, const double *y, double rho, double k) {
+                          int order, const double *y, double rho, double k, bool grouped) {
   auto C = [&](int i) { return y[idx[i]] * rho * invmw[i]; };
   switch (order) {
     case OR_ONE: return k * C(0);
-    case OR_TWO: return k * C(0) * C(0);
-    case OR_ONE_ONE: return k * C(0) * C(1);
-    case OR_ONE_ONE_ONE: return k * C(0) * C(1) * C(2);
-    case OR_TWO_ONE: return k * C(0) * C(0) * C(1);
-    case OR_ONE_TWO: return k * C(0) * C(1) * C(1);
+    case OR_TWO: return grouped ? k * (C(0) * C(0)) : k * C(0) * C(0);
+    case OR_ONE_ONE: return grouped ? k * (C(0) * C(1)) : k * C(0) * C(1);
+    case OR_ONE_ONE_ONE: return grouped ? k * (C(0) * C(1) * C(2)) : k * C(0) * C(1) * C(2);
+    case OR_TWO_ONE: return grouped ? k * (C(0) * C(0) * C(1)) : k * C(0) * C(0) * C(1);
+    case OR_ONE_TWO: return grouped ? k * (C(0) * C(1) * C(1)) : k * C(0) * C(1) * C(1);
     default:
       for (size_t i = 0; i < idx.size(); ++i) {
         switch (std::abs(st[i])) {
@@ -350,8 +353,8 @@ void production_rates(const Mech &m, double T, double rho, double mmw, const dou
         for (size_t i = 0; i < r.net_idx.size(); ++i) sg += r.net_st[i] * G[r.net_idx[i]];
         kr = k * std::exp(r.sumStoich * std::log(port) - invT * invRu * sg);
       }
-      k = mass_action(r.rc_idx, r.rc_st, r.rc_invmw, r.fwdOrder, y, rho, k);
-      if (r.reversible) kr = mass_action(r.pd_idx, r.pd_st, r.pd_invmw, r.revOrder, y, rho, kr);
+      k = mass_action(r.rc_idx, r.rc_st, r.rc_invmw, r.fwdOrder, y, rho, k, true);
+      if (r.reversible) kr = mass_action(r.pd_idx, r.pd_st, r.pd_invmw, r.revOrder, y, rho, kr, true);
     }
     for (size_t i = 0; i < r.net_idx.size(); ++i) w[r.net_idx[i]] -= r.net_st[i] * r.net_mw[i] * (k - kr);
   }
@@ -445,7 +448,7 @@ void prod_rates_sens_exact(const Mech &m, double T, double rho, double Mmix, con
         for (int s = 0; s < ns - 1; ++s) dRdY[s] -= t;
       }
     } else {
-      Rnet = mass_action(r.rc_idx, r.rc_st, r.rc_invmw, r.fwdOrder, y, rho, kf);
+      Rnet = mass_action(r.rc_idx, r.rc_st, r.rc_invmw, r.fwdOrder, y, rho, kf, false);
       dRdrho = Rnet / ct * invM * r.sumRcStoich;
       dRdT = Rnet * arrh_sens_over_k(r.kf, invT);
       int nsIdx = -1;
@@ -468,7 +471,7 @@ void prod_rates_sens_exact(const Mech &m, double T, double rho, double Mmix, con
           dKcdToverKc = -sb;
         }
         const double kr = kf / Kc;
-        const double Rr = mass_action(r.pd_idx, r.pd_st, r.pd_invmw, r.revOrder, y, rho, kr);
+        const double Rr = mass_action(r.pd_idx, r.pd_st, r.pd_invmw, r.revOrder, y, rho, kr, false);
         Rnet -= Rr;
         dRdrho -= Rr / ct * invM * r.sumPdStoich;
         dRdT -= Rr * (arrh_sens_over_k(r.kf, invT) - dKcdToverKc);

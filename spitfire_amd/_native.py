@@ -1,0 +1,88 @@
+"""Loader for libgriffon_amd.so (the C-ABI over the gfx950 HIP kernels).
+
+The library is built in-tree by spitfire_amd.build. There is no fallback: if it is missing or fails to load,
+every compute entry point raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, '_lib', 'libgriffon_amd.so')
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_L = ctypes.POINTER(ctypes.c_long)
+_V = ctypes.c_void_p
+_int, _dbl, _long = ctypes.c_int, ctypes.c_double, ctypes.c_long
+
+# (name, argtypes) for every symbol declared in include/griffon_amd.h
+SIGNATURES = {
+    'gf_last_error': [],
+    'gf_device_count': [],
+    'gf_mech_create': [ctypes.POINTER(_V)],
+    'gf_mech_destroy': [_V],
+    'gf_mechanism_set_gas_constant': [_V, _dbl],
+    'gf_mechanism_set_ref_pressure': [_V, _dbl],
+    'gf_mechanism_set_ref_temperature': [_V, _dbl],
+    'gf_mechanism_add_species': [_V, _dbl],
+    'gf_mechanism_resize_heat_capacity_data': [_V],
+    'gf_mechanism_add_const_cp': [_V, _int, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl],
+    'gf_mechanism_add_nasa7_cp': [_V, _int, _dbl, _dbl, _dbl, _D, _D],
+    'gf_mechanism_add_reaction': [_V, _int, _int, _int, _I, _I, _int, _I, _I, _D, _int, _I, _D, _dbl, _D, _D, _int,
+                                  _I, _D],
+    'gf_mech_n_species': [_V],
+    'gf_mech_n_reactions': [_V],
+    'gf_production_rates': [_V, _dbl, _dbl, _D, _D],
+    'gf_prod_rates_primitive_sensitivities': [_V, _dbl, _dbl, _D, _int, _D],
+    'gf_reactor_rhs_isobaric': [_V, _D, _dbl, _dbl, _D, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _int, _D],
+    'gf_reactor_jac_isobaric': [_V, _D, _dbl, _dbl, _D, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _int, _int, _int,
+                                _D, _D],
+    'gf_flamelet_stencils': [_V, _D, _int, _D, _D, _D, _D, _D, _D, _D],
+    'gf_flamelet_jac_indices': [_V, _int, _I, _I],
+    'gf_flamelet_rhs': [_V, _D, _dbl, _D, _D, _int, _D, _D, _D, _D, _int, _D, _D, _D, _D, _D, _D, _int, _int, _int,
+                        _D],
+    'gf_flamelet_jacobian': [_V, _D, _dbl, _D, _D, _int, _D, _D, _D, _D, _int, _D, _D, _D, _D, _D, _D, _int, _dbl,
+                             _int, _dbl, _int, _int, _int, _int, _int, _D, _D],
+    'gf_btddod_full_factorize': [_D, _int, _int, _D, _I],
+    'gf_btddod_full_solve': [_D, _D, _I, _D, _int, _int, _D],
+    'gf_btddod_scale_and_add_diagonal': [_D, _dbl, _D, _dbl, _int, _int],
+    'gf_batch_reactor_rhs_isobaric': [_V, _int, _V, _long, _long, _V, _long, _V, _long, _long, _V],
+    'gf_batch_reactor_jac_isobaric': [_V, _int, _V, _long, _long, _V, _long, _V, _long, _long, _V, _long, _V],
+    'gf_batch_btddod_full_factorize': [_int, _V, _int, _V, _V, _V, _V, _V, _V, _V],
+    'gf_batch_btddod_full_solve': [_int, _V, _int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V],
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load (once) and return the native library; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(LIB_PATH + ' is not built; run `python -m spitfire_amd.build`')
+        l = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(l, name)
+            f.argtypes = args
+            f.restype = ctypes.c_char_p if name == 'gf_last_error' else ctypes.c_int
+        _lib = l
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().gf_last_error().decode()
+        raise NativeError('griffon_amd error %d: %s' % (rc, msg))
+
+
+def dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_I)

@@ -1,0 +1,168 @@
+// Internal declarations shared by the griffon_amd host code and HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <cstdint>
+
+namespace gfa {
+
+constexpr int NS_MAX = 64;     // species per mechanism handled by the wave-per-state kernels (one lane per species)
+constexpr int NRC_MAX = 4;     // reactants per reaction
+constexpr int NPD_MAX = 4;     // products per reaction
+constexpr int NNET_MAX = 8;    // net species per reaction (griffon: 2..8)
+constexpr int NSP_MAX = 4;     // species with special (non-elementary) orders per reaction
+
+enum { CP_CONST = 1, CP_NASA7 = 2 };
+enum { RK_SIMPLE = 1, RK_THIRD_BODY = 2, RK_LINDEMANN = 3, RK_TROE = 4 };
+enum { KF_CONSTANT = 0, KF_LINEAR, KF_QUADRATIC, KF_RECIPROCAL, KF_ARRHENIUS };
+enum { TF_NONE = 0, TF_T1, TF_T2, TF_T12, TF_T3, TF_T13, TF_T23, TF_T123 };
+
+// One reaction as the kernels read it (48 + 88 bytes). Indices are species indices; stoichiometric
+// coefficients are positive for both sides. `slot` lists map each participant to its slot in the
+// per-state derivative vector (-1: the last species, which enters only through uniform corrections).
+struct DevRxn {
+  double kf[3];
+  double kp[3];
+  double troe[4];
+  double baseEff;
+  int type, kForm, troeForm;
+  int reversible, hasOrders;
+  int n_rc, n_pd, n_net, n_tb, n_sp;
+  int sumStoich, sumRc, sumPd;
+  int tb_off, sp_off, slot_off, n_slot;
+  int8_t tb_last;                 // index into the tb list of the last species, or -1
+  int8_t pad[3];
+  uint8_t rc_idx[NRC_MAX], rc_st[NRC_MAX];
+  uint8_t pd_idx[NPD_MAX], pd_st[NPD_MAX];
+  uint8_t net_idx[NNET_MAX];
+  int8_t net_st[NNET_MAX];
+  int8_t rc_slot[NRC_MAX], pd_slot[NPD_MAX];
+  int8_t sp_slot[NSP_MAX];
+};
+
+// Device view of a finalized mechanism (all pointers into one device allocation).
+struct MechView {
+  int ns, nr, nslot;
+  int last_const;           // highest constant-cp species index (-1: none); see cp_sens_T quirk
+  double Ru, invRu, pref;
+  const double *spc_c;      // [ns*16] griffon-packed NASA7/const coefficients (chemistry_setup.cpp:89-130)
+  const double *invmw;      // [ns]
+  const double *mw;         // [ns]
+  const double *dinvm;      // [ns] invmw[s] - invmw[ns-1]
+  const double *Tmin, *Tmax;
+  const int *cpkind;        // [ns]
+  const DevRxn *rx;         // [nr]
+  const int *tb_idx;        // third-body lists
+  const double *tb_eff;     //   invmw*(eff - default)  (chemistry_setup.cpp:416)
+  const int *tb_slot;       //   slot of each third body (-1 for the last species)
+  const int *sp_idx;        // special-order lists
+  const double *sp_ord;
+  const double *sp_invmw;
+  // species <- reactions gather (net stoichiometry): w_k = sum coef * q_r
+  const int *sg_off;        // [ns+1]
+  const int *sg_r;
+  const double *sg_coef;    // -net_stoich * net_mw  (rates_sensitivities_exact.cpp:1018)
+  // sensitivity entry (species row i < ns, species column j < ns-1) <- slots
+  const int *eg_off;        // [(ns-1)*ns+1], entry index e = j*ns + i
+  const int *eg_slot;
+  const double *eg_coef;
+  // temperature row, column j <- slots with species j (weight: the reaction's enthalpy change)
+  const int *cg_off;        // [ns]
+  const int *cg_slot;
+  const int *cg_r;
+};
+
+struct Species {
+  double mw = 0.;
+  int cpkind = 0;
+  double Tmin = 0., Tmax = 0.;
+  double c[16] = {0};
+};
+
+struct HostRxn {
+  int type = RK_SIMPLE;
+  bool reversible = true;
+  std::vector<int> rc_idx, rc_st, pd_idx, pd_st, tb_idx, sp_idx;
+  std::vector<double> tb_eff_raw, sp_ord;
+  double default_eff = 0.;
+  double kf[3] = {0, 0, 0}, kp[3] = {0, 0, 0}, troe[4] = {0, 0, 0, 0};
+};
+
+}  // namespace gfa
+
+struct gf_mech {
+  double Ru = 8314.46261815324;
+  double pref = 101325.;
+  double Tref = 298.15;
+  std::vector<gfa::Species> species;
+  std::vector<gfa::HostRxn> rxns;
+  // finalized host tables
+  bool finalized = false;
+  std::vector<gfa::DevRxn> drx;
+  std::vector<char> blob;         // host image of the device tables
+  gfa::MechView view_host;        // offsets (pointers relative to blob base)
+  int device = -1;
+  void *dblob = nullptr;          // device copy
+  gfa::MechView view;             // device pointers
+  size_t lds_per_state = 0;       // bytes of LDS the chemistry kernels need per state
+  // staging buffers for the host-pointer drop-ins
+  void *stage = nullptr;
+  size_t stage_bytes = 0;
+  hipStream_t stream = nullptr;
+};
+
+namespace gfa {
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int ensure_device(gf_mech *m);                 // finalize + upload on first compute call
+int ensure_stage(gf_mech *m, size_t bytes);
+
+// Inputs/outputs of one batched chemistry launch (gf_chem.hip). Element v of state i is x[i*xs_i + v*xs_v];
+// p[i*ps] is its pressure (or its density when rho_given, in which case x holds [T, y_0..y_{ns-1}]).
+struct ChemIO {
+  int n = 0;
+  const double *x = nullptr;
+  long xs_i = 0, xs_v = 0;
+  const double *p = nullptr;
+  long ps = 0;
+  int rho_given = 0;
+  double *rhs = nullptr;   // rhs[i*rs_i + v*rs_v]
+  long rs_i = 0, rs_v = 0;
+  double *jac = nullptr;   // ns*ns column-major block at jac + i*js_i
+  long js_i = 0;
+  double *wout = nullptr;  // net mass production rates [n][ns]
+  double *sens = nullptr;  // raw option-0 sensitivities [n][(ns+1)^2]
+  double *aux = nullptr;   // per state {rho, cp, dcp/dT, mixture molecular weight}
+  double *cpi = nullptr;   // per state species heat capacities [n][ns]
+};
+int launch_chem(const gf_mech *m, const ChemIO &io, hipStream_t st);
+
+// flamelet kernels (gf_flamelet.hip)
+struct FlameletIO {
+  int nzi = 0, ns = 0;
+  const double *state = nullptr;   // [nzi*ns] interior state (point-major)
+  const double *oxy = nullptr, *fuel = nullptr;
+  const double *cmaj = nullptr, *csub = nullptr, *csup = nullptr, *mco = nullptr, *nco = nullptr, *chi = nullptr;
+  int adiabatic = 1;
+  const double *Tc = nullptr, *hc = nullptr, *Tr = nullptr, *hr = nullptr;  // heat loss (C++ order)
+  int incl_ef = 0, incl_vcp = 0, scaled_hl = 0;
+  // per-point chemistry outputs over nzi+2 points (oxidizer, interior..., fuel)
+  const double *chem_rhs = nullptr;  // [(nzi+2)*ns]
+  const double *aux = nullptr;       // [(nzi+2)*4]
+  const double *cpi = nullptr;       // [(nzi+2)*ns]
+};
+int launch_flamelet_rhs_epilogue(const FlameletIO &f, double *out, hipStream_t st);
+int launch_flamelet_jac_epilogue(const gf_mech *m, const FlameletIO &f, int scale_and_offset, double prefactor,
+                                 double *jac, hipStream_t st);
+int launch_flamelet_stencils(int ns, const double *dz, int nzi, const double *chi, const double *invLe, double *cmaj,
+                             double *csub, double *csup, double *mco, double *nco, hipStream_t st);
+int launch_flamelet_jac_indices(int ns, int nzi, int *rows, int *cols, hipStream_t st);
+int launch_scale_and_add_diagonal(double *v, double a, const double *dg, double b, int nb, int bs, hipStream_t st);
+// block-tridiagonal kernels (gf_btd.hip)
+int launch_btd_factor(int nsys, const int *nblocks, int bs, double *values, const long *voff, double *l_values,
+                      const long *loff, int *pivots, const long *poff, hipStream_t st);
+int launch_btd_solve(int nsys, const int *nblocks, int bs, const double *values, const long *voff,
+                     const double *l_values, const long *loff, const int *pivots, const long *poff,
+                     const double *rhs, double *x, const long *xoff, hipStream_t st);
+}  // namespace gfa

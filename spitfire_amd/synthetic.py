@@ -16,7 +16,8 @@ Rate parameters follow the workload definitions: A log-uniform in [1e6, 1e14] in
 monotone seeded cp(T) curves, least-squares fitted by NASA7 polynomials on [300, 1000] K and
 [1000, 3500] K with cp, h and s continuous at 1000 K. Formation enthalpies follow a seeded
 bond-energy model so that H2/O2 mixtures are exothermic (H2O-like products are stable).
-Every reaction conserves elements, so mass production rates sum to zero.
+Every reaction conserves elements, so mass production rates sum to zero, and is written in its exothermic
+direction (recombinations as A + B (+M) -> C), so reverse rates from equilibrium constants stay bounded.
 """
 import itertools
 import numpy as np
@@ -120,6 +121,7 @@ def make_mechanism(n_species=53, n_reactions=325, n_three_body=None, n_troe=None
     mw = [sum(c[j] * ELEMENT_MW[e] for j, e in enumerate('HON')) for c in comps]
 
     species = {}
+    h298_of = []
     for i, (nm, c) in enumerate(zip(names, comps)):
         natoms = sum(c)
         if c in _STANDARD:
@@ -128,6 +130,7 @@ def make_mechanism(n_species=53, n_reactions=325, n_three_body=None, n_troe=None
             atomization = sum(c[j] * _ATOMIZATION[e] for j, e in enumerate('HON'))
             nbonds = max(natoms - 1, 0)
             h298 = atomization - sum(rng.uniform(380.0, 520.0) for _ in range(nbonds))
+        h298_of.append(h298)
         s298 = 14.0 + 4.0 * (natoms - 1) + rng.uniform(-1.0, 3.0)
         low, high = _nasa7_fit(rng, natoms, h298, s298)
         species[nm] = {'atoms': _ordered({e: float(c[j]) for j, e in enumerate('HON') if c[j] > 0}),
@@ -145,6 +148,12 @@ def make_mechanism(n_species=53, n_reactions=325, n_three_body=None, n_troe=None
 
     def arr(order):
         return cgs_A(order), rng.uniform(-1.5, 1.5), rng.uniform(0.0, 200.0) * 1.0e6  # Ea J/kmol
+
+    def exothermic(reac, prod):
+        """Write the reaction in its exothermic direction, so that kr = kf/Kc stays bounded by the
+        forward Arrhenius form (Ea - dG >= 0)."""
+        dH = sum(h298_of[i] for i in prod) - sum(h298_of[i] for i in reac)
+        return (reac, prod) if dH <= 0. else (prod, reac)
 
     def stoich_map(idxs):
         m = {}
@@ -181,6 +190,7 @@ def make_mechanism(n_species=53, n_reactions=325, n_three_body=None, n_troe=None
                 continue
             b, c = recomb_pairs[rng.integers(len(recomb_pairs))]
             reac, prod = [by_comp[add(b, c)]], [b, c]
+        reac, prod = exothermic(reac, prod)
         A, bexp, Ea = arr(len(reac))
         rxns.append({'type': 'simple', 'reversible': True, 'reactants': stoich_map(reac),
                      'products': stoich_map(prod), 'A': A, 'b': bexp, 'Ea': Ea})

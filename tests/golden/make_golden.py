@@ -1,0 +1,153 @@
+"""Write the golden fixtures in tests/golden/ from the reference's own griffon build (oracle/_ref).
+
+Run in a checkout where /root/reference exists (`make -C oracle ref` builds the reference sources unmodified
+with ref_harness.cpp). The fixtures are inputs and reference outputs only; the mechanisms are regenerated
+from their seeds by spitfire_amd.synthetic and checked against the stored table hash.
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle  # noqa: E402
+from spitfire_amd import synthetic  # noqa: E402
+from spitfire_amd.mechdata import CompiledMechanism  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def mech_hash(cm):
+    h = hashlib.sha256()
+    h.update(np.asarray(cm.mw).tobytes())
+    for th in cm.thermo:
+        for v in th[1:]:
+            h.update(np.asarray(v, dtype=np.float64).tobytes())
+    for r in cm.reactions:
+        for k in sorted(r):
+            v = r[k]
+            if isinstance(v, tuple):
+                for a in v:
+                    h.update(np.asarray(a).tobytes())
+            else:
+                h.update(np.asarray(v).tobytes())
+    return h.hexdigest()
+
+
+def chem_states(ns, n_random, seed, max_edge=None):
+    st, p, _ = synthetic.random_states(ns, n_random, seed=seed)
+    extra_T, extra_Y = [], []
+    # edge cases in the style of tests/griffon/test_reaction_rates.py:22-34 plus the NASA7 range ends
+    for k in range(ns if max_edge is None else min(ns, max_edge)):
+        y = np.full(ns, 1.e-8)
+        y[k] = 1.
+        extra_Y.append(y / y.sum())
+        extra_T.append([300., 1000., 1800., 2500.][k % 4])
+    y = np.full(ns, 1.e-16)
+    y[0] = 1.
+    extra_Y.append(y / y.sum())
+    extra_T.append(1200.)
+    for T in (250., 1000., 3500., 4000.):  # below Tmin, at Tmid, at Tmax, above Tmax
+        extra_Y.append(np.ones(ns) / ns)
+        extra_T.append(T)
+    Ye = np.array(extra_Y)
+    se = np.vstack([np.array(extra_T), Ye[:, :-1].T])
+    pe = np.full(len(extra_T), 101325.)
+    return np.hstack([st, se]), np.concatenate([p, pe])
+
+
+def chem_fixture(name, mech, n_random, seed, max_edge=None):
+    cm = CompiledMechanism(mech, order='index')
+    ref = oracle.CpuGriffon(cm, ref=True)
+    ns = cm.ns
+    x, p = chem_states(ns, n_random, seed, max_edge)
+    N = x.shape[1]
+    rhs = np.zeros((N, ns))
+    jac_rhs = np.zeros((N, ns))
+    jac = np.zeros((N, ns * ns))
+    w = np.zeros((N, ns))
+    sens = np.zeros((N, (ns + 1) ** 2))
+    rho = np.zeros(N)
+    for i in range(N):
+        rhs[i] = ref.reactor_rhs_isobaric(x[:, i], p[i])
+        jac_rhs[i], jac[i] = ref.reactor_jac_isobaric(x[:, i], p[i])
+        y = np.append(x[1:, i], 1. - np.sum(x[1:, i]))
+        mmw = 1. / np.sum(y / cm.mw)
+        rho[i] = p[i] * mmw / (x[0, i] * cm.gas_constant)
+        w[i] = ref.production_rates(x[0, i], rho[i], y)
+        sens[i] = ref.prod_rates_sens(rho[i], x[0, i], y)
+    np.savez_compressed(os.path.join(HERE, name), seed=mech['seed'], ns=ns, nr=cm.nr, mech_hash=mech_hash(cm),
+                        x=x, p=p, rho=rho, rhs=rhs, jac_rhs=jac_rhs, jac=jac, w=w, sens=sens)
+    print(name, N, 'states')
+
+
+def flamelet_case(ns, nzi, seed):
+    """A small flamelet: clustered-ish grid, Peters-type chi, smooth T/Y profiles between two streams."""
+    rng = np.random.default_rng(seed)
+    z = np.linspace(0., 1., nzi + 2) ** 1.3
+    dz = z[1:] - z[:-1]
+    chi = 5. * np.exp(-2. * (np.linspace(-1.5, 1.5, nzi + 2)) ** 2)
+    Yo = rng.dirichlet(np.ones(ns))
+    Yf = rng.dirichlet(np.ones(ns))
+    oxy = np.hstack([300., Yo[:-1]])
+    fuel = np.hstack([350., Yf[:-1]])
+    zi = z[1:-1]
+    state = np.zeros((nzi, ns))
+    for i in range(nzi):
+        Y = (1 - zi[i]) * Yo + zi[i] * Yf
+        T = 300. + 1600. * np.exp(-((zi[i] - 0.3) / 0.2) ** 2)
+        state[i, 0] = T
+        state[i, 1:] = Y[:-1]
+    return dz, chi, oxy, fuel, state.ravel()
+
+
+def flamelet_fixture(name, mech, nzi, seed, variants=((0, 0), (0, 1), (1, 0), (1, 1))):
+    cm = CompiledMechanism(mech, order='index')
+    ref = oracle.CpuGriffon(cm, ref=True)
+    ns = cm.ns
+    dz, chi, oxy, fuel, state = flamelet_case(ns, nzi, seed)
+    p = 101325.
+    st = ref.flamelet_stencils(dz, nzi, chi, np.ones(ns))
+    out = dict(seed=mech['seed'], ns=ns, nzi=nzi, mech_hash=mech_hash(cm), dz=dz, chi=chi, oxy=oxy, fuel=fuel,
+               state=state, p=p, cmaj=st[0], csub=st[1], csup=st[2], mco=st[3], nco=st[4])
+    rows, cols = ref.flamelet_jac_indices(nzi)
+    out['rows'], out['cols'] = rows, cols
+    for ef, vcp in variants:
+        out['rhs_%d%d' % (ef, vcp)] = ref.flamelet_rhs(state, p, oxy, fuel, nzi, st, chi, incl_ef=ef, incl_vcp=vcp)
+        out['jac_%d%d' % (ef, vcp)] = ref.flamelet_jacobian(state, p, oxy, fuel, nzi, st, chi, incl_ef=ef,
+                                                            incl_vcp=vcp)[0]
+    out['jac_so'] = ref.flamelet_jacobian(state, p, oxy, fuel, nzi, st, chi, scale_and_offset=True,
+                                          prefactor=-2.5e-4)[0]
+    jac_eig, expeig = ref.flamelet_jacobian(state, p, oxy, fuel, nzi, st, chi, compute_eig=True, diffterm=4. * 5.)
+    out['expeig'] = expeig
+    # the pseudo-transient matrix diag(1/ds) - J (flamelet.py:1599-1606) with ds = 0.1 / (expeig + 1),
+    # built by btddod_scale_and_add_diagonal, then factored and solved by block Thomas
+    diag = (np.max(expeig) + 1.) / 0.1 * (1. + 0.5 * np.abs(np.cos(np.arange(nzi * ns))))
+    out['sad_diag'] = diag
+    out['sad'] = ref.btddod_scale_and_add_diagonal(out['jac_00'], -1., diag, 1., nzi, ns)
+    A = out['sad']
+    d, l, piv = ref.btddod_full_factorize(A, nzi, ns)
+    b = np.sin(np.arange(nzi * ns) + 1.)
+    out['btd_A'], out['btd_d'], out['btd_l'], out['btd_piv'] = A, d, l, piv
+    out['btd_b'], out['btd_x'] = b, ref.btddod_full_solve(d, l, piv, b, nzi, ns)
+    np.savez_compressed(os.path.join(HERE, name), **out)
+    print(name, nzi, 'interior points')
+
+
+def main():
+    oracle.build(ref=True)
+    if not oracle.ref_available():
+        raise SystemExit('oracle/_ref was not built (needs the reference checkout)')
+    chem_fixture('chem_c1.npz', synthetic.mechanism_c1(), 48, seed=11)
+    chem_fixture('chem_53.npz', synthetic.mechanism_53(), 12, seed=12, max_edge=8)
+    flamelet_fixture('flamelet_c1.npz', synthetic.mechanism_c1(), 12, seed=13)
+    flamelet_fixture('flamelet_53.npz', synthetic.mechanism_53(), 5, seed=14, variants=((0, 0), (1, 1)))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,167 @@
+"""GPU parity of the chemistry kernels against the oracle and the reference fixtures (through the C-ABI).
+
+Tolerances (FP64): RHS / production rates and Jacobian entries are compared per row in the normwise sense,
+max_j |gpu_ij - ref_ij| / max_j |ref_ij| <= 1e-12 (a row of the Jacobian is one equation's gradient; the
+kernel sums the same terms in a different order than griffon, so elementwise relative error is bounded
+only where a sum does not cancel). Elementwise relative errors are also reported and bounded where the
+reference entry is not dominated by cancellation (|ref_ij| >= 1e-6 max_j |ref_ij|).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests.helpers import row_norm_err
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+RTOL = 1e-12
+ELEM_RTOL = 1e-9
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _spec(data):
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    return ChemicalMechanismSpec(data)
+
+
+def _rows_err(a, b):
+    a = np.atleast_2d(a)
+    b = np.atleast_2d(b)
+    num = np.max(np.abs(a - b), axis=1)
+    den = np.max(np.abs(b), axis=1)
+    return float(np.max(num / np.maximum(den, 1e-300)))
+
+
+def _elem_err(a, b, n):
+    """elementwise relative error over entries not dominated by cancellation, for column-major n x n blocks"""
+    A = np.asarray(a).reshape(-1, n, n)
+    B = np.asarray(b).reshape(-1, n, n)
+    rowmax = np.max(np.abs(B), axis=1, keepdims=True)
+    mask = np.abs(B) >= 1e-6 * rowmax
+    e = np.abs(A - B) / np.maximum(np.abs(B), 1e-300)
+    return float(np.max(np.where(mask, e, 0.)))
+
+
+@pytest.mark.parametrize('fx, mech', [('chem_c1.npz', 'mech_c1_data'), ('chem_53.npz', 'mech_53_data')])
+def test_dropins_match_reference_fixtures(fx, mech, request):
+    f = np.load(os.path.join(GOLD, fx))
+    spec = _spec(request.getfixturevalue(mech))
+    g = spec.griffon
+    ns = spec.n_species
+    x, p = f['x'], f['p']
+    say('drop-in fixtures', fx, x.shape[1], 'states')
+    errs = dict(rhs=0., jac=0., jac_elem=0., w=0., sens=0.)
+    null = np.zeros(1)
+    for i in range(x.shape[1]):
+        st = np.ascontiguousarray(x[:, i])
+        rhs = np.zeros(ns)
+        g.reactor_rhs_isobaric(st, p[i], 0., null, 0., 0., 0., 0., 0., 0., 0, False, rhs)
+        jr, jac = np.zeros(ns), np.zeros(ns * ns)
+        g.reactor_jac_isobaric(st, p[i], 0., null, 0., 0., 0., 0., 0., 0., 0, False, 0, 0, jr, jac)
+        y = np.append(x[1:, i], 1. - np.sum(x[1:, i]))
+        w = np.zeros(ns)
+        g.production_rates(x[0, i], f['rho'][i], y, w)
+        sens = np.zeros((ns + 1) ** 2)
+        g.prod_rates_primitive_sensitivities(f['rho'][i], x[0, i], y, 0, sens)
+        errs['rhs'] = max(errs['rhs'], _rows_err(rhs, f['rhs'][i]), _rows_err(jr, f['jac_rhs'][i]))
+        errs['w'] = max(errs['w'], _rows_err(w, f['w'][i]))
+        errs['jac'] = max(errs['jac'], row_norm_err(jac, f['jac'][i], ns))
+        errs['jac_elem'] = max(errs['jac_elem'], _elem_err(jac, f['jac'][i], ns))
+        S = sens.reshape(ns + 1, ns + 1)[:, :ns]  # [col, row]; last row unused
+        R = f['sens'][i].reshape(ns + 1, ns + 1)[:, :ns]
+        errs['sens'] = max(errs['sens'], float(np.max(np.max(np.abs(S - R), axis=0) /
+                                                      np.maximum(np.max(np.abs(R), axis=0), 1e-300))))
+    say('errors', errs)
+    assert errs['rhs'] <= RTOL and errs['w'] <= RTOL and errs['jac'] <= RTOL and errs['sens'] <= RTOL
+    assert errs['jac_elem'] <= ELEM_RTOL
+
+
+def _batched(spec, x, p, layout='soa'):
+    import torch
+    ns = spec.n_species
+    N = x.shape[1]
+    dev = torch.device('cuda:0')
+    xt = torch.tensor(x if layout == 'soa' else np.ascontiguousarray(x.T), device=dev)
+    pt = torch.tensor(p, device=dev)
+    rhs = torch.empty_like(xt)
+    jac = torch.empty((N, ns * ns), dtype=torch.float64, device=dev)
+    spec.griffon.batch_reactor_jac_isobaric(xt, pt, rhs, jac, layout=layout)
+    torch.cuda.synchronize()
+    r = rhs.cpu().numpy()
+    return (r.T if layout == 'soa' else r), jac.cpu().numpy()
+
+
+def test_batched_matches_oracle_c2_distribution(mech_53_data, oracle_lib):
+    from spitfire_amd import synthetic
+    from spitfire_amd.mechdata import CompiledMechanism
+    spec = _spec(mech_53_data)
+    ns = spec.n_species
+    x, p, _ = synthetic.random_states(ns, 2048, seed=2024)
+    say('batched 53 species, 2048 states')
+    rhs, jac = _batched(spec, x, p)
+    o = oracle_lib.CpuGriffon(CompiledMechanism(mech_53_data, order='name'))
+    orhs, ojac = o.batch_rhs_jac(x, p, nthreads=0)
+    e_rhs = _rows_err(rhs, orhs)
+    e_jac = row_norm_err(jac, ojac, ns)
+    e_el = _elem_err(jac, ojac, ns)
+    say('rhs row err %.3e  jac row err %.3e  jac elem err %.3e' % (e_rhs, e_jac, e_el))
+    assert e_rhs <= RTOL and e_jac <= RTOL and e_el <= ELEM_RTOL
+    # array-of-structures layout gives bit-identical results
+    rhs2, jac2 = _batched(spec, x, p, layout='aos')
+    assert np.array_equal(rhs, rhs2) and np.array_equal(jac, jac2)
+
+
+def test_batched_deterministic_and_rhs_only_kernel_agrees(mech_53_data):
+    import torch
+    from spitfire_amd import synthetic
+    spec = _spec(mech_53_data)
+    ns = spec.n_species
+    x, p, _ = synthetic.random_states(ns, 1000, seed=7)
+    say('determinism, 1000 states')
+    r1, j1 = _batched(spec, x, p)
+    r2, j2 = _batched(spec, x, p)
+    assert np.array_equal(r1, r2) and np.array_equal(j1, j2)
+    dev = torch.device('cuda:0')
+    xt = torch.tensor(x, device=dev)
+    rr = torch.empty_like(xt)
+    spec.griffon.batch_reactor_rhs_isobaric(xt, torch.tensor(p, device=dev), rr)
+    torch.cuda.synchronize()
+    # production_rates formulation vs the sensitivity formulation of the rates (griffon computes both)
+    assert _rows_err(rr.cpu().numpy().T, r1) <= RTOL
+
+
+def test_full_size_batch_sampled_parity(mech_53_data, oracle_lib):
+    """The 262,144-state workload: every state finite, two launches identical, a sample against the oracle."""
+    import torch
+    from spitfire_amd import synthetic
+    from spitfire_amd.mechdata import CompiledMechanism
+    spec = _spec(mech_53_data)
+    ns = spec.n_species
+    N = 262144
+    x, p, _ = synthetic.random_states(ns, N, seed=0)
+    say('full-size batch', N, 'states')
+    dev = torch.device('cuda:0')
+    xt = torch.tensor(x, device=dev)
+    pt = torch.tensor(p, device=dev)
+    rhs = torch.empty_like(xt)
+    jac = torch.empty((N, ns * ns), dtype=torch.float64, device=dev)
+    spec.griffon.batch_reactor_jac_isobaric(xt, pt, rhs, jac)
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(jac).all()) and bool(torch.isfinite(rhs).all())
+    c1 = float(jac.sum()), float(jac.abs().sum())
+    jac2 = torch.empty_like(jac)
+    spec.griffon.batch_reactor_jac_isobaric(xt, pt, rhs, jac2)
+    torch.cuda.synchronize()
+    assert bool(torch.equal(jac, jac2))
+    idx = np.random.default_rng(1).choice(N, 256, replace=False)
+    o = oracle_lib.CpuGriffon(CompiledMechanism(mech_53_data, order='name'))
+    orhs, ojac = o.batch_rhs_jac(np.ascontiguousarray(x[:, idx]), np.ascontiguousarray(p[idx]))
+    gj = jac[torch.tensor(idx, device=dev)].cpu().numpy()
+    gr = rhs[:, torch.tensor(idx, device=dev)].cpu().numpy().T
+    e = row_norm_err(gj, ojac, ns)
+    say('sampled jac row err %.3e rhs %.3e checksum %r' % (e, _rows_err(gr, orhs), c1))
+    assert e <= RTOL and _rows_err(gr, orhs) <= RTOL

@@ -1,0 +1,145 @@
+"""Benchmark: batched isobaric RHS + analytic Jacobian evaluations per second on one MI355X.
+
+Workload (the headline RHS+Jac configuration): the seeded synthetic 53-species / 325-reaction H/O/N mechanism
+evaluated over 262,144 independent states (T uniform in 300-2500 K, p log-uniform in 0.1-10 atm, Y ~
+Dirichlet(0.5)); state in structure-of-arrays layout, RHS out in the same layout, one column-major 53 x 53
+FP64 Jacobian per state. One step = one batched evaluation of all states (one kernel launch); inputs are
+resident in HBM before the timed region.
+
+Prints one JSON line with the throughput, the HBM roofline fraction of the kernel (algorithmic bytes per
+evaluation = 8 * (ns + 1 + ns + ns^2) = 23,328 B: state and pressure in, RHS and Jacobian out) and the
+reference's own griffon build (oracle/_ref) timed on this host's cores on a bounded sample as the CPU
+baseline.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_STATES = 262144
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'chem_jac_53sp_262144_traffic.json')
+
+
+def cpu_baseline(data, x, p, sample, threads):
+    """Reference griffon (oracle/_ref) if built, else the oracle restatement; OpenMP over states."""
+    import numpy as np
+    from oracle import oracle
+    from spitfire_amd.mechdata import CompiledMechanism
+    ref = oracle.ref_available()
+    cm = CompiledMechanism(data, order='index' if ref else 'name')
+    cpu = oracle.CpuGriffon(cm, ref=ref)
+    xs = np.ascontiguousarray(x[:, :sample])
+    ps = np.ascontiguousarray(p[:sample])
+    cpu.batch_rhs_jac(xs[:, :256], ps[:256], nthreads=threads)  # warm-up
+    t0 = time.perf_counter()
+    cpu.batch_rhs_jac(xs, ps, nthreads=threads)
+    dt = time.perf_counter() - t0
+    n1 = max(256, sample // 64)
+    t0 = time.perf_counter()
+    cpu.batch_rhs_jac(np.ascontiguousarray(xs[:, :n1]), np.ascontiguousarray(ps[:n1]), nthreads=1)
+    dt1 = time.perf_counter() - t0
+    return {'value': sample / dt, 'unit': 'RHS+Jac evals/s', 'cores': threads,
+            'kind': 'reference' if ref else 'port',
+            'sample': '%d states of the same workload (first %d of the batch), OpenMP over states; '
+                      'single-core rate %.4g evals/s on %d states' % (sample, sample, n1 / dt1, n1),
+            'single_core_value': n1 / dt1}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--states', type=int, default=N_STATES)
+    ap.add_argument('--cpu-sample', type=int, default=131072)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--copy-calibration-bytes', type=int, default=0,
+                    help='run one device-to-device copy of this many bytes first (counter calibration under rocprofv3)')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build runs the batched chemistry on a single MI355X; '
+              'independent states shard across GPUs without communication (see DESIGN.md)', file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from spitfire_amd import synthetic
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+
+    data = synthetic.mechanism_53()
+    spec = ChemicalMechanismSpec(data)
+    ns, nr = spec.n_species, spec.n_reactions
+    N = args.states
+    x, p, _ = synthetic.random_states(ns, N, seed=0)
+    dev = torch.device('cuda:0')
+    xt = torch.tensor(x, device=dev)
+    pt = torch.tensor(p, device=dev)
+    rhs = torch.empty_like(xt)
+    jac = torch.empty((N, ns * ns), dtype=torch.float64, device=dev)
+    g = spec.griffon
+    stream = torch.cuda.current_stream()
+
+    if args.copy_calibration_bytes:
+        n8 = args.copy_calibration_bytes // 8
+        a = torch.ones(n8, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    for _ in range(args.warmup):
+        g.batch_reactor_jac_isobaric(xt, pt, rhs, jac, stream=stream)
+    torch.cuda.synchronize()
+    print('warmup done', file=sys.stderr, flush=True)
+
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        g.batch_reactor_jac_isobaric(xt, pt, rhs, jac, stream=stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    value = N * args.steps / wall
+    if not bool(torch.isfinite(jac).all()):
+        raise RuntimeError('non-finite Jacobian entries')
+
+    bytes_per_eval = 8 * (ns + 1 + ns + ns * ns)
+    achieved = bytes_per_eval * N / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if t.get('states') == N:
+            traffic = t.get('bytes_per_launch')
+    out = {
+        'metric': 'RHS+Jac evals/s (53 sp) & flamelet solves/s, FP32-LU vs FP64 at 1e-8',
+        'value': value, 'unit': 'RHS+Jac evals/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64', 'data': 'synthetic',
+        'config': {'workload': 'batched isobaric reactor RHS + analytic Jacobian, %d species / %d reactions, '
+                               '%d states (SoA state in, column-major ns x ns Jacobian per state out)' % (ns, nr, N),
+                   'n_species': ns, 'n_reactions': nr, 'states': N, 'mechanism_seed': data['seed'],
+                   'state_seed': 0},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
+                     'kernel': 'chem_kernel (RHS + Jacobian)', 'kernel_ms': kernel_ms,
+                     'algorithmic_bytes_per_eval': bytes_per_eval},
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        out['cpu_baseline'] = cpu_baseline(data, x, p, min(args.cpu_sample, N), threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -308,13 +308,17 @@ inline double mass_action(const std::vector<int> &idx, const std::vector<int> &s
 }
 
 // chemistry_kernels.cpp:35-463 — net mass production rates
-void production_rates(const Mech &m, double T, double rho, double mmw, const double *y, double *w) {
+// wabs (optional) receives sum_r |nu_kr M_k| (|forward rate| + |reverse rate|), the magnitude of the terms whose
+// signed sum is w_k; test tolerances for sums are stated relative to it.
+void production_rates(const Mech &m, double T, double rho, double mmw, const double *y, double *w,
+                      double *wabs = nullptr) {
   const double invT = 1 / T;
   const double logT = std::log(T);
   const double conc = rho / mmw;
   const double invRu = 1. / m.Ru;
   const double port = m.pref * invT * invRu;
   for (int i = 0; i < m.ns; ++i) w[i] = 0.;
+  if (wabs) for (int i = 0; i < m.ns; ++i) wabs[i] = 0.;
   std::vector<double> G(m.ns);
   species_gibbs(m, T, logT, invT, G.data(), nullptr);
 
@@ -356,7 +360,10 @@ void production_rates(const Mech &m, double T, double rho, double mmw, const dou
       k = mass_action(r.rc_idx, r.rc_st, r.rc_invmw, r.fwdOrder, y, rho, k, true);
       if (r.reversible) kr = mass_action(r.pd_idx, r.pd_st, r.pd_invmw, r.revOrder, y, rho, kr, true);
     }
-    for (size_t i = 0; i < r.net_idx.size(); ++i) w[r.net_idx[i]] -= r.net_st[i] * r.net_mw[i] * (k - kr);
+    for (size_t i = 0; i < r.net_idx.size(); ++i) {
+      if (wabs) wabs[r.net_idx[i]] += std::abs(r.net_st[i] * r.net_mw[i]) * (std::abs(k) + std::abs(kr));
+      w[r.net_idx[i]] -= r.net_st[i] * r.net_mw[i] * (k - kr);
+    }
   }
 }
 
@@ -875,6 +882,32 @@ int gfo_n_reactions(void *h) { return (int)static_cast<Mech *>(h)->rx.size(); }
 int gfo_production_rates(void *h, double T, double rho, const double *y, double *w) {
   const Mech &m = *static_cast<Mech *>(h);
   try { production_rates(m, T, rho, mmw_of(m, y), y, w); } catch (...) { return 2; }
+  return 0;
+}
+
+// Magnitude scale of the isobaric RHS over N states (state SoA): scale[i*ns + 0] = sum_k |h_k| wabs_k / (rho cp),
+// scale[i*ns + 1 + k] = wabs_k / rho, with wabs from production_rates.
+int gfo_batch_rhs_scale(void *h, int N, const double *stateSoA, const double *p, double *scale) {
+  const Mech &m = *static_cast<Mech *>(h);
+  const int n = m.ns;
+  std::vector<double> st(n), y(n), hh(n), w(n), wa(n), cpi(n);
+  try {
+    for (int i = 0; i < N; ++i) {
+      for (int v = 0; v < n; ++v) st[v] = stateSoA[(size_t)v * N + i];
+      const double T = st[0];
+      extract_y(m, &st[1], y.data());
+      const double mmw = mmw_of(m, y.data());
+      const double rho = p[i] * mmw / (T * m.Ru);
+      double cp;
+      cp_mix_and_species(m, T, y.data(), &cp, cpi.data());
+      species_enthalpies(m, T, hh.data());
+      production_rates(m, T, rho, mmw, y.data(), w.data(), wa.data());
+      double s0 = 0.;
+      for (int k = 0; k < n; ++k) s0 += std::abs(hh[k]) * wa[k];
+      scale[(size_t)i * n] = s0 / (rho * cp);
+      for (int k = 0; k < n - 1; ++k) scale[(size_t)i * n + 1 + k] = wa[k] / rho;
+    }
+  } catch (...) { return 2; }
   return 0;
 }
 

@@ -164,6 +164,15 @@ class CpuGriffon:
         assert self._f('batch_rhs_jac')(self.h, N, _dp(st), _dp(pp), _dp(rhs), _dp(jac), nthreads) == 0
         return rhs, jac
 
+    def batch_rhs_scale(self, stateSoA, p):
+        """Per-entry magnitude of the terms summed into the RHS (oracle only): |dT/dt| and |dY/dt| scales."""
+        assert not self.ref
+        N = stateSoA.shape[1]
+        sc = np.empty((N, self.ns))
+        assert self._f('batch_rhs_scale')(self.h, N, _dp(np.ascontiguousarray(stateSoA, dtype=np.float64)),
+                                          _dp(np.ascontiguousarray(p, dtype=np.float64)), _dp(sc)) == 0
+        return sc
+
     def batch_rhs(self, stateSoA, p, nthreads=0):
         N = stateSoA.shape[1]
         rhs = np.empty((N, self.ns))

@@ -60,11 +60,20 @@ class NativeError(RuntimeError):
 
 
 def lib():
-    """Load (once) and return the native library; raises if it is not built."""
+    """Load (once) and return the native library; raises if it is not built.
+
+    PyTorch-ROCm bundles its own HIP/HSA runtime (torch/lib/libamdhip64.so, SONAME libamdhip64.so.7). torch is
+    imported first so that the library's libamdhip64.so.7 dependency resolves to that already-loaded runtime:
+    a process must hold exactly one HSA runtime, or the second one finds no GPU.
+    """
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise NativeError(LIB_PATH + ' is not built; run `python -m spitfire_amd.build`')
+        try:
+            import torch  # noqa: F401  (loads torch's HIP runtime before ours)
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
             f = getattr(l, name)

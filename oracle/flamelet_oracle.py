@@ -1,0 +1,64 @@
+"""TEST INFRASTRUCTURE ONLY — a Flamelet whose RHS, Jacobian and block-tridiagonal algebra run on the CPU
+oracle (oracle/griffon_oracle.cpp) instead of the GPU: the checker for the steady-flamelet solves and the CPU
+baseline of the flamelet benchmark. The solver logic (Newton / pseudo-transient continuation) is the product's
+spitfire_amd.flamelet.Flamelet, unchanged."""
+import numpy as np
+
+from oracle.oracle import CpuGriffon
+from spitfire_amd.flamelet import Flamelet
+from spitfire_amd.mechdata import CompiledMechanism
+
+
+class OracleFlamelet(Flamelet):
+    def __init__(self, flamelet_specs=None, cpu=None, **kwargs):
+        spec = flamelet_specs if flamelet_specs is not None else kwargs
+        mech = spec.mech_spec if hasattr(spec, 'mech_spec') else spec['mech_spec']
+        self._cpu = cpu if cpu is not None else CpuGriffon(CompiledMechanism(mech.mech_data, order='name'))
+        super().__init__(flamelet_specs, **kwargs)
+
+    def _setup_stencils(self):
+        st = self._cpu.flamelet_stencils(self._dz, self._nz_interior, self._x, 1. / self._lewis_numbers)
+        (self._maj_coeff_griffon, self._sub_coeff_griffon, self._sup_coeff_griffon, self._mcoeff_griffon,
+         self._ncoeff_griffon) = st
+
+    def _stencils(self):
+        return (self._maj_coeff_griffon, self._sub_coeff_griffon, self._sup_coeff_griffon, self._mcoeff_griffon,
+                self._ncoeff_griffon)
+
+    def _adiabatic_rhs(self, t, state_interior):
+        return self._cpu.flamelet_rhs(np.ascontiguousarray(state_interior), self._pressure, self._state_oxy,
+                                      self._state_fuel, self._nz_interior, self._stencils(), self._x,
+                                      incl_ef=self._include_enthalpy_flux, incl_vcp=self._include_variable_cp)
+
+    def _adiabatic_jac(self, state_interior, scale_and_offset=False, prefactor=0.):
+        return self._cpu.flamelet_jacobian(np.ascontiguousarray(state_interior), self._pressure, self._state_oxy,
+                                           self._state_fuel, self._nz_interior, self._stencils(), self._x,
+                                           scale_and_offset=scale_and_offset, prefactor=prefactor,
+                                           incl_ef=self._include_enthalpy_flux,
+                                           incl_vcp=self._include_variable_cp)[0]
+
+    def _factorize(self, values):
+        d, l, piv = self._cpu.btddod_full_factorize(values, self._nz_interior, self._n_equations)
+        values[:] = d
+        self._block_thomas_l_values = l
+        self._block_thomas_d_pivots = piv
+
+    def _solve(self, values, rhs):
+        return self._cpu.btddod_full_solve(values, self._block_thomas_l_values, self._block_thomas_d_pivots, rhs,
+                                           self._nz_interior, self._n_equations)
+
+    def _scale_add_diag(self, values, a, diag, b):
+        values[:] = self._cpu.btddod_scale_and_add_diagonal(values, a, diag, b, self._nz_interior,
+                                                           self._n_equations)
+
+
+class OracleFlameletEig(OracleFlamelet):
+    """The reference's pseudo-transient step limiter: expeig from LAPACK dgeev of every chemistry block
+    (flamelet_kernels.cpp:1329-1341, through numpy.linalg.eigvals)."""
+
+    def eigenvalue_bound(self, state_interior, diffterm):
+        return self._cpu.flamelet_jacobian(np.ascontiguousarray(state_interior), self._pressure, self._state_oxy,
+                                           self._state_fuel, self._nz_interior, self._stencils(), self._x,
+                                           compute_eig=True, diffterm=diffterm,
+                                           incl_ef=self._include_enthalpy_flux,
+                                           incl_vcp=self._include_variable_cp)[1]

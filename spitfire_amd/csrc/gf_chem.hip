@@ -5,20 +5,26 @@
 // chemistry_kernels.cpp:35-463 (production rates), rates_sensitivities_exact.cpp:33-1028 (exact
 // sensitivities), isobaric_reactor_kernels.cpp:19-98, 319-343 (RHS, primitive Jacobian, transform).
 //
-// Structure (all per-state intermediates live in LDS; no atomics, so results are deterministic):
+// Structure: persistent workgroups (LDS-limited, about one per CU), each owning a contiguous range of
+// states processed S at a time; all per-state intermediates live in LDS; no atomics, so results are
+// deterministic:
 //   P0  one wave per state, lane = species: mixture thermo; wave-shuffle reductions for sum(y/M), cp, dcp/dT
-//   P1  lane = (state, reaction): kf, Kc, falloff factor, mass action and all partial derivatives.
+//   P1  lane = (reaction, state), a reaction's S states in adjacent lanes: kf, Kc, falloff factor, mass
+//       action and all partial derivatives.
 //       dq_r/dY_s = sum_p v_{r,p} delta(s,p) + a_r (1/M_s - 1/M_last) + b_r, so each reaction stores five
 //       scalars (q, dq/drho, dq/dT, a, b), its enthalpy change H_r and one value v per participant slot.
 //   P2  one wave per state, lane = species k: gathers over the reactions with k in their net
 //       stoichiometry give w_k, dw_k/drho, dw_k/dT and the rank-1 coefficients; wave reductions give the
 //       enthalpy-weighted sums for the temperature row; the RHS is written here.
-//   P3  lane = Jacobian entry (column-major, so stores are contiguous): sparse gather over the slots of
-//       that (row species, column species) pair + rank-1 part + chem_jac_isobaric + the (rho,T,Y)->(T,Y)
-//       transform at constant pressure.
+//   P3a lane = (column, state): the temperature-row and diagonal sums of each column (the diagonal entry
+//       collects every reaction of that species, so it is summed here rather than in the entry pass).
+//   P3b lane = Jacobian entry (column-major, so stores are contiguous), looping over the S states: sparse
+//       gather over the slots of that (row species, column species) pair + rank-1 part + chem_jac_isobaric
+//       + the (rho,T,Y)->(T,Y) transform at constant pressure.
 #include "gf_internal.h"
 #include "../../include/griffon_amd.h"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -27,10 +33,10 @@ namespace gfa {
 namespace {
 
 constexpr int SP_STRIDE = NS_MAX;   // per-species LDS arrays
-constexpr int N_SPEC_ARR = 11;      // y h cpi G dBdT w ws0 ws1 A B pr
+constexpr int N_SPEC_ARR = 13;      // y h cpi G dBdT w ws0 ws1 A B pr SH DG
 constexpr int N_SCAL = 16;
 
-enum { Y_ = 0, H_, CPI_, G_, DBDT_, W_, WS0_, WS1_, A_, B_, PR_ };
+enum { Y_ = 0, H_, CPI_, G_, DBDT_, W_, WS0_, WS1_, A_, B_, PR_, SH_, DG_ };
 enum { S_T = 0, S_RHO, S_MMW, S_CP, S_CPST, S_INVT, S_LOGT, S_CT, S_INVM, S_LOGPORT, S_RHS0, S_HS0, S_HS1, S_WCP,
        S_HA, S_HB };
 
@@ -116,375 +122,417 @@ __device__ __forceinline__ double troe_fcent(const DevRxn &R, double T, double i
 }
 
 template <int S, int THREADS, bool JAC>
-__global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a) {
+__global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, long groups_per_wg) {
   extern __shared__ double lds[];
   const int ns = mv.ns, nr = mv.nr, nm = ns - 1;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const long stride = N_SPEC_ARR * SP_STRIDE + N_SCAL + 6L * nr + mv.nslot;
+  const long ngroups = (a.n + S - 1) / S;
+  // each workgroup owns a contiguous range of state groups (neighbouring states share SoA input lines)
+  const long g0 = (long)blockIdx.x * groups_per_wg;
+  const long g1 = min(ngroups, g0 + groups_per_wg);
 
-  // XCD-aware group order: consecutive state groups go to workgroups of the same XCD (blockIdx % 8),
-  // so neighbouring states' strided SoA inputs share that XCD's L2 lines.
-  const int G = gridDim.x;
-  const int b = blockIdx.x;
-  const int per = G / 8, rem = G % 8, xcd = b % 8, kk = b / 8;
-  const long group = (long)xcd * per + min(xcd, rem) + kk;
-  const long first = group * S;
+  for (long group = g0; group < g1; ++group) {
+    const long first = group * S;
 
-  // ---------------- P0: mixture thermo (wave per state) ----------------
-  if (wave < S) {
-    const long i = first + wave;
-    double *sp = lds + wave * stride;
-    double *sc = sp + N_SPEC_ARR * SP_STRIDE;
-    if (i < a.n) {
-      const double *xi = a.x + i * a.xs_i;
-      const double T = xi[0];
-      double yk = 0.;
-      if (a.rho_given) {
-        if (lane < ns) yk = xi[(1 + lane) * a.xs_v];
-      } else {
-        if (lane < nm) yk = xi[(1 + lane) * a.xs_v];
-        const double sy = wave_sum(yk);
-        if (lane == nm) yk = 1. - sy;
-      }
-      const double mm = wave_sum(lane < ns ? yk * mv.invmw[lane] : 0.);
-      const double mmw = 1. / mm;
-      const double invT = 1. / T, logT = log(T);
-      const double rho = a.rho_given ? a.p[i * a.ps] : a.p[i * a.ps] * mmw / (T * mv.Ru);
-      double cp = 0., cps = 0., h = 0., Gk = 0., dB = 0.;
-      if (lane < ns) species_thermo(mv, lane, T, logT, invT, cp, cps, h, Gk, dB);
-      if (lane < ns) {
-        sp[Y_ * SP_STRIDE + lane] = yk;
-        sp[H_ * SP_STRIDE + lane] = h;
-        sp[CPI_ * SP_STRIDE + lane] = cp;
-        sp[G_ * SP_STRIDE + lane] = Gk;
-        sp[DBDT_ * SP_STRIDE + lane] = dB;
-      }
-      const double cpm = wave_sum(lane < ns ? yk * cp : 0.);
-      // a constant-cp species resets the dcp/dT accumulation (thermodynamics_kernels.cpp:200-203)
-      const double cpsm = wave_sum((lane < ns && lane > mv.last_const) ? yk * cps : 0.);
-      if (a.aux && lane < 4) a.aux[i * 4 + lane] = lane == 0 ? rho : (lane == 1 ? cpm : (lane == 2 ? cpsm : mmw));
-      if (a.cpi && lane < ns) a.cpi[i * ns + lane] = cp;
-      if (lane == 0) {
-        sc[S_T] = T; sc[S_RHO] = rho; sc[S_MMW] = mmw; sc[S_CP] = cpm; sc[S_CPST] = cpsm;
-        sc[S_INVT] = invT; sc[S_LOGT] = logT; sc[S_CT] = rho * mm; sc[S_INVM] = mm;
-        sc[S_LOGPORT] = log(mv.pref * invT * mv.invRu);
-      }
-    }
-  }
-  __syncthreads();
-
-  // ---------------- P1: reactions ----------------
-  for (int it = threadIdx.x; it < S * nr; it += THREADS) {
-    const int s = it / nr, r = it - s * nr;
-    const long i = first + s;
-    if (i >= a.n) continue;
-    double *sp = lds + s * stride;
-    const double *sc = sp + N_SPEC_ARR * SP_STRIDE;
-    double *rq = sp + N_SPEC_ARR * SP_STRIDE + N_SCAL;
-    double *v = rq + 6 * nr;
-    const double *y = sp + Y_ * SP_STRIDE;
-    const double T = sc[S_T], rho = sc[S_RHO], invT = sc[S_INVT], logT = sc[S_LOGT], ct = sc[S_CT],
-                 invM = sc[S_INVM];
-    const DevRxn &R = mv.rx[r];
-    const double kf = kf_of(R, T, logT, invT);
-
-    // third-body / falloff multiplier
-    double tbsum = 0.;
-    for (int q = 0; q < R.n_tb; ++q) tbsum += mv.tb_eff[R.tb_off + q] * y[mv.tb_idx[R.tb_off + q]];
-
-    if (!JAC) {
-      // production_rates formulation (chemistry_kernels.cpp:140-461)
-      double k = kf, kr = 0.;
-      const double M = R.baseEff * ct + rho * tbsum;
-      if (R.type == RK_THIRD_BODY) {
-        k *= M;
-      } else if (R.type == RK_LINDEMANN) {
-        k /= (1 + k / (arrh(R.kp, logT, invT) * M));
-      } else if (R.type == RK_TROE) {
-        const double pr = arrh(R.kp, logT, invT) / k * M;
-        double dd;
-        const double logFc = log10(troe_fcent(R, T, invT, dd));
-        const double logPrC = log10(fmax(pr, 1.e-300)) + (-0.4 - 0.67 * logFc);
-        const double F1 = logPrC / ((0.75 - 1.27 * logFc) - 0.14 * logPrC);
-        k = k * pow(10., logFc / (1 + F1 * F1)) * pr / (1 + pr);
-      }
-      if (R.hasOrders) {
-        for (int q = 0; q < R.n_sp; ++q) {
-          const int sidx = mv.sp_idx[R.sp_off + q];
-          k *= pow(fmax(y[sidx] * rho * mv.sp_invmw[R.sp_off + q], 0.), mv.sp_ord[R.sp_off + q]);
-        }
-      } else {
-        if (R.reversible) {
-          double sg = 0.;
-          for (int q = 0; q < R.n_net; ++q) sg += R.net_st[q] * sp[G_ * SP_STRIDE + R.net_idx[q]];
-          kr = k * exp(R.sumStoich * sc[S_LOGPORT] - invT * mv.invRu * sg);
-        }
-        for (int j = 0; j < R.n_rc; ++j) {
-          const int sidx = R.rc_idx[j];
-          k *= ipow3(y[sidx] * rho * mv.invmw[sidx], R.rc_st[j]);
-        }
-        if (R.reversible)
-          for (int j = 0; j < R.n_pd; ++j) {
-            const int sidx = R.pd_idx[j];
-            kr *= ipow3(y[sidx] * rho * mv.invmw[sidx], R.pd_st[j]);
-          }
-      }
-      rq[r] = k - kr;
-      continue;
-    }
-
-    double Ctbaf = 1., dCdrho = 0., dCdT = 0., cdense = 0., tbs = 0.;
-    if (R.type == RK_THIRD_BODY) {
-      Ctbaf = R.baseEff * ct + rho * tbsum;
-      dCdrho = R.baseEff * invM + tbsum;
-      cdense = rho * R.baseEff;
-      tbs = rho;
-    } else if (R.type == RK_LINDEMANN || R.type == RK_TROE) {
-      const double flf = R.baseEff * ct + rho * tbsum;
-      const double kpk = arrh(R.kp, logT, invT) / kf;
-      const double pr = kpk * flf;
-      const double dlnk = arrh_sens_over_k(R.kp, invT) - arrh_sens_over_k(R.kf, invT);
-      double nsT;
-      if (R.type == RK_LINDEMANN) {
-        Ctbaf = pr / (1. + pr);
-        dCdT = Ctbaf / (1. + pr) * dlnk;
-        nsT = kpk / ((1. + pr) * (1. + pr));
-      } else {  // rates_sensitivities_exact.cpp:905-995
-        double dfc;
-        const double fc = troe_fcent(R, T, invT, dfc);
-        const double l10pr = log10(fmax(pr, 1.e-300));
-        const double l10fc = log10(fmax(fc, 1.e-300));
-        const double lfc = log(fmax(fc, 1.e-300));
-        const double ln10 = 2.302585092994045684;  // log(10)
-        const double aT = l10pr - 0.67 * l10fc - 0.4;
-        const double bT = -0.14 * l10pr - 1.1762 * l10fc + 0.806;
-        const double gT = 1 / (1 + (aT / bT) * (aT / bT));
-        const double fT = pow(fc, gT);
-        Ctbaf = fT * pr / (1 + pr);
-        const double dfTdT = fT * (gT / fc * dfc + lfc * (-2.0 * gT * gT / ln10 * aT / (bT * bT * bT) *
-                                                            ((bT + 0.14 * aT) * dlnk - (0.67 * bT - 1.1762 * aT) * dfc / fc)));
-        dCdT = 1. / (1. + 1. / pr) * dfTdT + fT * pr / ((1. + pr) * (1. + pr)) * dlnk;
-        nsT = kpk * (-2.0 / (1. + pr) * fT * lfc * gT * gT / ln10 * aT / (bT * bT * bT) * (bT + 0.14 * aT) +
-                     fT / ((1. + pr) * (1 + pr)));
-      }
-      dCdrho = nsT * R.baseEff * invM + nsT * tbsum;
-      cdense = nsT * rho * R.baseEff;
-      tbs = nsT * rho;
-    }
-
-    double *vr = v + R.slot_off;
-    for (int q = 0; q < R.n_slot; ++q) vr[q] = 0.;
-    double Rnet, dRdrho, dRdT, U = 0.;
-    const double ksens = arrh_sens_over_k(R.kf, invT);
-    if (R.hasOrders) {  // rates_sensitivities_exact.cpp:198-281
-      double C[NSP_MAX], ordv[NSP_MAX];
-      int sidx[NSP_MAX];
-      Rnet = kf;
-      double sumO = 0.;
-      for (int q = 0; q < R.n_sp; ++q) {
-        sidx[q] = mv.sp_idx[R.sp_off + q];
-        ordv[q] = mv.sp_ord[R.sp_off + q];
-        C[q] = y[sidx[q]] * rho * mv.sp_invmw[R.sp_off + q];
-        Rnet *= pow(fmax(C[q], 0.), ordv[q]);
-        sumO += ordv[q];
-      }
-      dRdrho = Rnet / ct * invM * sumO;
-      dRdT = Rnet * ksens;
-      for (int j = 0; j < R.n_sp; ++j) {
-        const double im = mv.sp_invmw[R.sp_off + j];
-        const bool last = (sidx[j] == ns - 1);
-        double m = kf;
-        for (int l = 0; l < R.n_sp; ++l) {
-          if (l != j) {
-            m *= last ? pow(C[l], ordv[l]) : pow(fmax(C[l], 0.), ordv[l]);
-          } else {
-            if (last || ordv[l] > 1) m *= ordv[l] * rho * im * pow(fmax(C[l], 1.e-16), ordv[l] - 1.);
-            else m *= ordv[l] * rho * im / pow(fmax(C[l], 1.e-16), 1. - ordv[l]);
-          }
-        }
-        if (last) U -= m;
-        else vr[R.sp_slot[j]] += Ctbaf * m;
-      }
-    } else {
-      double C[NRC_MAX];
-      Rnet = kf;
-      for (int j = 0; j < R.n_rc; ++j) {
-        C[j] = y[R.rc_idx[j]] * rho * mv.invmw[R.rc_idx[j]];
-        Rnet *= ipow3(C[j], R.rc_st[j]);
-      }
-      dRdrho = Rnet / ct * invM * R.sumRc;
-      dRdT = Rnet * ksens;
-      for (int j = 0; j < R.n_rc; ++j) {
-        double d = kf * rho * mv.invmw[R.rc_idx[j]] * R.rc_st[j] * (R.rc_st[j] == 1 ? 1. : ipow3(C[j], R.rc_st[j] - 1));
-        for (int l = 0; l < R.n_rc; ++l)
-          if (l != j) d *= ipow3(C[l], R.rc_st[l]);
-        if (R.rc_slot[j] >= 0) vr[R.rc_slot[j]] += Ctbaf * d;
-        else U -= d;
-      }
-      if (R.reversible) {
-        double sg = 0., sb = 0.;
-        for (int q = 0; q < R.n_net; ++q) {
-          sg += R.net_st[q] * sp[G_ * SP_STRIDE + R.net_idx[q]];
-          sb += R.net_st[q] * sp[DBDT_ * SP_STRIDE + R.net_idx[q]];
-        }
-        const double Kc = exp(-(R.sumStoich * sc[S_LOGPORT] - invT * mv.invRu * sg));
-        const double kr = kf / Kc;
-        double Cp[NPD_MAX];
-        double Rr = kr;
-        for (int j = 0; j < R.n_pd; ++j) {
-          Cp[j] = y[R.pd_idx[j]] * rho * mv.invmw[R.pd_idx[j]];
-          Rr *= ipow3(Cp[j], R.pd_st[j]);
-        }
-        Rnet -= Rr;
-        dRdrho -= Rr / ct * invM * R.sumPd;
-        dRdT -= Rr * (ksens + sb);
-        for (int j = 0; j < R.n_pd; ++j) {
-          double d = kr * rho * mv.invmw[R.pd_idx[j]] * R.pd_st[j] * (R.pd_st[j] == 1 ? 1. : ipow3(Cp[j], R.pd_st[j] - 1));
-          for (int l = 0; l < R.n_pd; ++l)
-            if (l != j) d *= ipow3(Cp[l], R.pd_st[l]);
-          if (R.pd_slot[j] >= 0) vr[R.pd_slot[j]] -= Ctbaf * d;
-          else U += d;
-        }
-      }
-    }
-    double Elast = 0.;
-    for (int q = 0; q < R.n_tb; ++q) {
-      const int slot = mv.tb_slot[R.tb_off + q];
-      const double e = mv.tb_eff[R.tb_off + q];
-      if (slot >= 0) vr[slot] += Rnet * tbs * e;
-      else if (q == R.tb_last) Elast = tbs * e;
-    }
-    double Hr = 0.;
-    for (int q = 0; q < R.n_net; ++q) {
-      const int k = R.net_idx[q];
-      Hr += (-R.net_st[q] * mv.mw[k]) * sp[H_ * SP_STRIDE + k];
-    }
-    rq[r] = Rnet * Ctbaf;
-    rq[nr + r] = dRdrho * Ctbaf + dCdrho * Rnet;
-    rq[2 * nr + r] = dRdT * Ctbaf + dCdT * Rnet;
-    rq[3 * nr + r] = Rnet * cdense;
-    rq[4 * nr + r] = Ctbaf * U - Rnet * Elast;
-    rq[5 * nr + r] = Hr;
-  }
-  __syncthreads();
-
-  // ---------------- P2: species gathers + reductions (wave per state) ----------------
-  if (wave < S) {
-    const long i = first + wave;
-    double *sp = lds + wave * stride;
-    double *sc = sp + N_SPEC_ARR * SP_STRIDE;
-    const double *rq = sc + N_SCAL;
-    if (i < a.n) {
-      double w = 0., w0 = 0., w1 = 0., A = 0., B = 0.;
-      if (lane < ns) {
-        const int e0 = mv.sg_off[lane], e1 = mv.sg_off[lane + 1];
-        for (int e = e0; e < e1; ++e) {
-          const int r = mv.sg_r[e];
-          const double c = mv.sg_coef[e];
-          w += c * rq[r];
-          if (JAC) {
-            w0 += c * rq[nr + r];
-            w1 += c * rq[2 * nr + r];
-            A += c * rq[3 * nr + r];
-            B += c * rq[4 * nr + r];
-          }
-        }
-      }
-      const double h = lane < ns ? sp[H_ * SP_STRIDE + lane] : 0.;
-      const double rho = sc[S_RHO], cp = sc[S_CP];
-      const double HW = wave_sum(h * w);
-      const double rhs0 = -HW / (rho * cp);
-      const double iR = 1. / rho;
-      if (a.rhs) {
-        double *ro = a.rhs + i * a.rs_i;
-        if (lane == 0) ro[0] = rhs0;
-        if (lane < nm) ro[(1 + lane) * a.rs_v] = w * iR;
-      }
-      if (a.wout && lane < ns) a.wout[i * ns + lane] = w;
-      if (JAC) {
-        const double cpi = lane < ns ? sp[CPI_ * SP_STRIDE + lane] : 0.;
-        const double HS0 = wave_sum(h * w0), HS1 = wave_sum(h * w1), WCP = wave_sum(w * cpi);
-        const double HA = wave_sum(h * A), HB = wave_sum(h * B);
-        if (lane < ns) {
-          sp[W_ * SP_STRIDE + lane] = w;
-          sp[WS0_ * SP_STRIDE + lane] = w0;
-          sp[WS1_ * SP_STRIDE + lane] = w1;
-          sp[A_ * SP_STRIDE + lane] = A;
-          sp[B_ * SP_STRIDE + lane] = B;
-        }
-        const double iRC = 1. / (rho * cp);
-        if (lane < nm) sp[PR_ * SP_STRIDE + 1 + lane] = iR * (w0 - iR * w);
-        if (lane == 0) {
-          sp[PR_ * SP_STRIDE] = -iRC * HS0 - iR * rhs0;
-          sc[S_RHS0] = rhs0; sc[S_HS0] = HS0; sc[S_HS1] = HS1; sc[S_WCP] = WCP; sc[S_HA] = HA; sc[S_HB] = HB;
-        }
-      }
-    }
-  }
-  if (!JAC) return;
-  __syncthreads();
-
-  // ---------------- P3: Jacobian entries ----------------
-  const int nent = ns * ns;
-  for (int s = 0; s < S; ++s) {
-    const long i = first + s;
-    if (i >= a.n) break;
-    const double *sp = lds + s * stride;
-    const double *sc = sp + N_SPEC_ARR * SP_STRIDE;
-    const double *rq = sc + N_SCAL;
-    const double *v = rq + 6 * nr;
-    const double T = sc[S_T], rho = sc[S_RHO], cp = sc[S_CP], mmw = sc[S_MMW];
-    const double iR = 1. / rho, iC = 1. / cp, iRC = 1. / (rho * cp), roT = rho / T;
-    const double rhs0 = sc[S_RHS0];
-    const double *pr = sp + PR_ * SP_STRIDE;
-    if (a.sens) {
-      // raw option-0 sensitivities, (ns+1)^2 column-major with stride ns+1 (rates_sensitivities_exact.cpp:1011-1026)
-      double *so = a.sens + i * (long)(ns + 1) * (ns + 1);
-      for (int e = threadIdx.x; e < (ns + 1) * (ns + 1); e += THREADS) {
-        const int c = e / (ns + 1), k = e - c * (ns + 1);
-        double val = 0.;
-        if (k < ns) {
-          if (c == 0) val = sp[WS0_ * SP_STRIDE + k];
-          else if (c == 1) val = sp[WS1_ * SP_STRIDE + k];
-          else {
-            const int j = c - 2;
-            const int ee = j * ns + k;
-            double spv = 0.;
-            for (int t = mv.eg_off[ee]; t < mv.eg_off[ee + 1]; ++t) spv += mv.eg_coef[t] * v[mv.eg_slot[t]];
-            val = spv + mv.dinvm[j] * sp[A_ * SP_STRIDE + k] + sp[B_ * SP_STRIDE + k];
-          }
-        }
-        so[e] = val;
-      }
-    }
-    if (!a.jac) continue;
-    double *jo = a.jac + i * a.js_i;
-    for (int e = threadIdx.x; e < nent; e += THREADS) {
-      const int c = e / ns, rr = e - c * ns;
-      double val;
-      if (c == 0) {
-        if (rr == 0) val = -iRC * (sc[S_HS1] + sc[S_WCP]) - rhs0 * sc[S_CPST] * iC - roT * pr[0];
-        else val = sp[WS1_ * SP_STRIDE + rr - 1] * iR - roT * pr[rr];
-      } else {
-        const int j = c - 1;
-        const double dj = mv.dinvm[j];
-        const double cj = -rho * mmw * dj;
-        if (rr == 0) {
-          double SH = 0.;
-          for (int t = mv.cg_off[j]; t < mv.cg_off[j + 1]; ++t) SH += v[mv.cg_slot[t]] * rq[5 * nr + mv.cg_r[t]];
-          val = -iRC * (SH + dj * sc[S_HA] + sc[S_HB]) -
-                rhs0 * (sp[CPI_ * SP_STRIDE + j] - sp[CPI_ * SP_STRIDE + nm]) * iC + cj * pr[0];
+    // ---------------- P0: mixture thermo (wave per state, lane = species) ----------------
+    if (wave < S) {
+      const long i = first + wave;
+      double *sp = lds + wave * stride;
+      double *sc = sp + N_SPEC_ARR * SP_STRIDE;
+      if (i < a.n) {
+        const double *xi = a.x + i * a.xs_i;
+        const double T = xi[0];
+        double yk = 0.;
+        if (a.rho_given) {
+          if (lane < ns) yk = xi[(1 + lane) * a.xs_v];
         } else {
-          const int ii = rr - 1;
-          const int ee = j * ns + ii;
-          double spv = 0.;
-          for (int t = mv.eg_off[ee]; t < mv.eg_off[ee + 1]; ++t) spv += mv.eg_coef[t] * v[mv.eg_slot[t]];
-          val = iR * (spv + dj * sp[A_ * SP_STRIDE + ii] + sp[B_ * SP_STRIDE + ii]) + cj * pr[rr];
+          if (lane < nm) yk = xi[(1 + lane) * a.xs_v];
+          const double sy = wave_sum(yk);
+          if (lane == nm) yk = 1. - sy;
+        }
+        const double mm = wave_sum(lane < ns ? yk * mv.invmw[lane] : 0.);
+        const double mmw = 1. / mm;
+        const double invT = 1. / T, logT = log(T);
+        const double rho = a.rho_given ? a.p[i * a.ps] : a.p[i * a.ps] * mmw / (T * mv.Ru);
+        double cp = 0., cps = 0., h = 0., Gk = 0., dB = 0.;
+        if (lane < ns) species_thermo(mv, lane, T, logT, invT, cp, cps, h, Gk, dB);
+        if (lane < ns) {
+          sp[Y_ * SP_STRIDE + lane] = yk;
+          sp[H_ * SP_STRIDE + lane] = h;
+          sp[CPI_ * SP_STRIDE + lane] = cp;
+          sp[G_ * SP_STRIDE + lane] = Gk;
+          sp[DBDT_ * SP_STRIDE + lane] = dB;
+        }
+        const double cpm = wave_sum(lane < ns ? yk * cp : 0.);
+        // a constant-cp species resets the dcp/dT accumulation (thermodynamics_kernels.cpp:200-203)
+        const double cpsm = wave_sum((lane < ns && lane > mv.last_const) ? yk * cps : 0.);
+        if (a.aux && lane < 4) a.aux[i * 4 + lane] = lane == 0 ? rho : (lane == 1 ? cpm : (lane == 2 ? cpsm : mmw));
+        if (a.cpi && lane < ns) a.cpi[i * ns + lane] = cp;
+        if (lane == 0) {
+          sc[S_T] = T; sc[S_RHO] = rho; sc[S_MMW] = mmw; sc[S_CP] = cpm; sc[S_CPST] = cpsm;
+          sc[S_INVT] = invT; sc[S_LOGT] = logT; sc[S_CT] = rho * mm; sc[S_INVM] = mm;
+          sc[S_LOGPORT] = log(mv.pref * invT * mv.invRu);
         }
       }
-      jo[e] = val;
     }
+    __syncthreads();
+
+    // ---------------- P1: reactions (lane = (reaction, state); a reaction's S states are adjacent lanes) -------
+    for (int it = threadIdx.x; it < S * nr; it += THREADS) {
+      const int r = it / S, s = it - r * S;
+      const long i = first + s;
+      if (i >= a.n) continue;
+      double *sp = lds + s * stride;
+      const double *sc = sp + N_SPEC_ARR * SP_STRIDE;
+      double *rq = sp + N_SPEC_ARR * SP_STRIDE + N_SCAL;
+      double *v = rq + 6 * nr;
+      const double *y = sp + Y_ * SP_STRIDE;
+      const double T = sc[S_T], rho = sc[S_RHO], invT = sc[S_INVT], logT = sc[S_LOGT], ct = sc[S_CT],
+                   invM = sc[S_INVM];
+      const DevRxn &R = mv.rx[r];
+      const double kf = kf_of(R, T, logT, invT);
+
+      // third-body / falloff multiplier
+      double tbsum = 0.;
+      for (int q = 0; q < R.n_tb; ++q) tbsum += mv.tb_eff[R.tb_off + q] * y[mv.tb_idx[R.tb_off + q]];
+
+      if (!JAC) {
+        // production_rates formulation (chemistry_kernels.cpp:140-461)
+        double k = kf, kr = 0.;
+        const double M = R.baseEff * ct + rho * tbsum;
+        if (R.type == RK_THIRD_BODY) {
+          k *= M;
+        } else if (R.type == RK_LINDEMANN) {
+          k /= (1 + k / (arrh(R.kp, logT, invT) * M));
+        } else if (R.type == RK_TROE) {
+          const double pr = arrh(R.kp, logT, invT) / k * M;
+          double dd;
+          const double logFc = log10(troe_fcent(R, T, invT, dd));
+          const double logPrC = log10(fmax(pr, 1.e-300)) + (-0.4 - 0.67 * logFc);
+          const double F1 = logPrC / ((0.75 - 1.27 * logFc) - 0.14 * logPrC);
+          k = k * pow(10., logFc / (1 + F1 * F1)) * pr / (1 + pr);
+        }
+        if (R.hasOrders) {
+          for (int q = 0; q < R.n_sp; ++q) {
+            const int sidx = mv.sp_idx[R.sp_off + q];
+            k *= pow(fmax(y[sidx] * rho * mv.sp_invmw[R.sp_off + q], 0.), mv.sp_ord[R.sp_off + q]);
+          }
+        } else {
+          if (R.reversible) {
+            double sg = 0.;
+            for (int q = 0; q < R.n_net; ++q) sg += R.net_st[q] * sp[G_ * SP_STRIDE + R.net_idx[q]];
+            kr = k * exp(R.sumStoich * sc[S_LOGPORT] - invT * mv.invRu * sg);
+          }
+          for (int j = 0; j < R.n_rc; ++j) {
+            const int sidx = R.rc_idx[j];
+            k *= ipow3(y[sidx] * rho * mv.invmw[sidx], R.rc_st[j]);
+          }
+          if (R.reversible)
+            for (int j = 0; j < R.n_pd; ++j) {
+              const int sidx = R.pd_idx[j];
+              kr *= ipow3(y[sidx] * rho * mv.invmw[sidx], R.pd_st[j]);
+            }
+        }
+        rq[r] = k - kr;
+        continue;
+      }
+
+      double Ctbaf = 1., dCdrho = 0., dCdT = 0., cdense = 0., tbs = 0.;
+      if (R.type == RK_THIRD_BODY) {
+        Ctbaf = R.baseEff * ct + rho * tbsum;
+        dCdrho = R.baseEff * invM + tbsum;
+        cdense = rho * R.baseEff;
+        tbs = rho;
+      } else if (R.type == RK_LINDEMANN || R.type == RK_TROE) {
+        const double flf = R.baseEff * ct + rho * tbsum;
+        const double kpk = arrh(R.kp, logT, invT) / kf;
+        const double pr = kpk * flf;
+        const double dlnk = arrh_sens_over_k(R.kp, invT) - arrh_sens_over_k(R.kf, invT);
+        double nsT;
+        if (R.type == RK_LINDEMANN) {
+          Ctbaf = pr / (1. + pr);
+          dCdT = Ctbaf / (1. + pr) * dlnk;
+          nsT = kpk / ((1. + pr) * (1. + pr));
+        } else {  // rates_sensitivities_exact.cpp:905-995
+          double dfc;
+          const double fc = troe_fcent(R, T, invT, dfc);
+          const double l10pr = log10(fmax(pr, 1.e-300));
+          const double l10fc = log10(fmax(fc, 1.e-300));
+          const double lfc = log(fmax(fc, 1.e-300));
+          const double ln10 = 2.302585092994045684;  // log(10)
+          const double aT = l10pr - 0.67 * l10fc - 0.4;
+          const double bT = -0.14 * l10pr - 1.1762 * l10fc + 0.806;
+          const double gT = 1 / (1 + (aT / bT) * (aT / bT));
+          const double fT = pow(fc, gT);
+          Ctbaf = fT * pr / (1 + pr);
+          const double dfTdT = fT * (gT / fc * dfc + lfc * (-2.0 * gT * gT / ln10 * aT / (bT * bT * bT) *
+                                                              ((bT + 0.14 * aT) * dlnk - (0.67 * bT - 1.1762 * aT) * dfc / fc)));
+          dCdT = 1. / (1. + 1. / pr) * dfTdT + fT * pr / ((1. + pr) * (1. + pr)) * dlnk;
+          nsT = kpk * (-2.0 / (1. + pr) * fT * lfc * gT * gT / ln10 * aT / (bT * bT * bT) * (bT + 0.14 * aT) +
+                       fT / ((1. + pr) * (1 + pr)));
+        }
+        dCdrho = nsT * R.baseEff * invM + nsT * tbsum;
+        cdense = nsT * rho * R.baseEff;
+        tbs = nsT * rho;
+      }
+
+      double *vr = v + R.slot_off;
+      for (int q = 0; q < R.n_slot; ++q) vr[q] = 0.;
+      double Rnet, dRdrho, dRdT, U = 0.;
+      const double ksens = arrh_sens_over_k(R.kf, invT);
+      if (R.hasOrders) {  // rates_sensitivities_exact.cpp:198-281
+        double C[NSP_MAX], ordv[NSP_MAX];
+        int sidx[NSP_MAX];
+        Rnet = kf;
+        double sumO = 0.;
+        for (int q = 0; q < R.n_sp; ++q) {
+          sidx[q] = mv.sp_idx[R.sp_off + q];
+          ordv[q] = mv.sp_ord[R.sp_off + q];
+          C[q] = y[sidx[q]] * rho * mv.sp_invmw[R.sp_off + q];
+          Rnet *= pow(fmax(C[q], 0.), ordv[q]);
+          sumO += ordv[q];
+        }
+        dRdrho = Rnet / ct * invM * sumO;
+        dRdT = Rnet * ksens;
+        for (int j = 0; j < R.n_sp; ++j) {
+          const double im = mv.sp_invmw[R.sp_off + j];
+          const bool last = (sidx[j] == ns - 1);
+          double m = kf;
+          for (int l = 0; l < R.n_sp; ++l) {
+            if (l != j) {
+              m *= last ? pow(C[l], ordv[l]) : pow(fmax(C[l], 0.), ordv[l]);
+            } else {
+              if (last || ordv[l] > 1) m *= ordv[l] * rho * im * pow(fmax(C[l], 1.e-16), ordv[l] - 1.);
+              else m *= ordv[l] * rho * im / pow(fmax(C[l], 1.e-16), 1. - ordv[l]);
+            }
+          }
+          if (last) U -= m;
+          else vr[R.sp_slot[j]] += Ctbaf * m;
+        }
+      } else {
+        double C[NRC_MAX];
+        Rnet = kf;
+        for (int j = 0; j < R.n_rc; ++j) {
+          C[j] = y[R.rc_idx[j]] * rho * mv.invmw[R.rc_idx[j]];
+          Rnet *= ipow3(C[j], R.rc_st[j]);
+        }
+        dRdrho = Rnet / ct * invM * R.sumRc;
+        dRdT = Rnet * ksens;
+        for (int j = 0; j < R.n_rc; ++j) {
+          double d = kf * rho * mv.invmw[R.rc_idx[j]] * R.rc_st[j] * (R.rc_st[j] == 1 ? 1. : ipow3(C[j], R.rc_st[j] - 1));
+          for (int l = 0; l < R.n_rc; ++l)
+            if (l != j) d *= ipow3(C[l], R.rc_st[l]);
+          if (R.rc_slot[j] >= 0) vr[R.rc_slot[j]] += Ctbaf * d;
+          else U -= d;
+        }
+        if (R.reversible) {
+          double sg = 0., sb = 0.;
+          for (int q = 0; q < R.n_net; ++q) {
+            sg += R.net_st[q] * sp[G_ * SP_STRIDE + R.net_idx[q]];
+            sb += R.net_st[q] * sp[DBDT_ * SP_STRIDE + R.net_idx[q]];
+          }
+          const double Kc = exp(-(R.sumStoich * sc[S_LOGPORT] - invT * mv.invRu * sg));
+          const double kr = kf / Kc;
+          double Cp[NPD_MAX];
+          double Rr = kr;
+          for (int j = 0; j < R.n_pd; ++j) {
+            Cp[j] = y[R.pd_idx[j]] * rho * mv.invmw[R.pd_idx[j]];
+            Rr *= ipow3(Cp[j], R.pd_st[j]);
+          }
+          Rnet -= Rr;
+          dRdrho -= Rr / ct * invM * R.sumPd;
+          dRdT -= Rr * (ksens + sb);
+          for (int j = 0; j < R.n_pd; ++j) {
+            double d = kr * rho * mv.invmw[R.pd_idx[j]] * R.pd_st[j] * (R.pd_st[j] == 1 ? 1. : ipow3(Cp[j], R.pd_st[j] - 1));
+            for (int l = 0; l < R.n_pd; ++l)
+              if (l != j) d *= ipow3(Cp[l], R.pd_st[l]);
+            if (R.pd_slot[j] >= 0) vr[R.pd_slot[j]] -= Ctbaf * d;
+            else U += d;
+          }
+        }
+      }
+      double Elast = 0.;
+      for (int q = 0; q < R.n_tb; ++q) {
+        const int slot = mv.tb_slot[R.tb_off + q];
+        const double e = mv.tb_eff[R.tb_off + q];
+        if (slot >= 0) vr[slot] += Rnet * tbs * e;
+        else if (q == R.tb_last) Elast = tbs * e;
+      }
+      double Hr = 0.;
+      for (int q = 0; q < R.n_net; ++q) {
+        const int k = R.net_idx[q];
+        Hr += (-R.net_st[q] * mv.mw[k]) * sp[H_ * SP_STRIDE + k];
+      }
+      rq[r] = Rnet * Ctbaf;
+      rq[nr + r] = dRdrho * Ctbaf + dCdrho * Rnet;
+      rq[2 * nr + r] = dRdT * Ctbaf + dCdT * Rnet;
+      rq[3 * nr + r] = Rnet * cdense;
+      rq[4 * nr + r] = Ctbaf * U - Rnet * Elast;
+      rq[5 * nr + r] = Hr;
+    }
+    __syncthreads();
+
+    // ---------------- P2: species gathers + reductions (wave per state, lane = species) ----------------
+    if (wave < S) {
+      const long i = first + wave;
+      double *sp = lds + wave * stride;
+      double *sc = sp + N_SPEC_ARR * SP_STRIDE;
+      const double *rq = sc + N_SCAL;
+      if (i < a.n) {
+        double w = 0., w0 = 0., w1 = 0., A = 0., B = 0.;
+        if (lane < ns) {
+          const int e0 = mv.sg_off[lane], e1 = mv.sg_off[lane + 1];
+          for (int e = e0; e < e1; ++e) {
+            const int r = mv.sg_r[e];
+            const double c = mv.sg_coef[e];
+            w += c * rq[r];
+            if (JAC) {
+              w0 += c * rq[nr + r];
+              w1 += c * rq[2 * nr + r];
+              A += c * rq[3 * nr + r];
+              B += c * rq[4 * nr + r];
+            }
+          }
+        }
+        const double h = lane < ns ? sp[H_ * SP_STRIDE + lane] : 0.;
+        const double rho = sc[S_RHO], cp = sc[S_CP];
+        const double HW = wave_sum(h * w);
+        const double rhs0 = -HW / (rho * cp);
+        const double iR = 1. / rho;
+        if (a.rhs) {
+          double *ro = a.rhs + i * a.rs_i;
+          if (lane == 0) ro[0] = rhs0;
+          if (lane < nm) ro[(1 + lane) * a.rs_v] = w * iR;
+        }
+        if (a.wout && lane < ns) a.wout[i * ns + lane] = w;
+        if (JAC) {
+          const double cpi = lane < ns ? sp[CPI_ * SP_STRIDE + lane] : 0.;
+          const double HS0 = wave_sum(h * w0), HS1 = wave_sum(h * w1), WCP = wave_sum(w * cpi);
+          const double HA = wave_sum(h * A), HB = wave_sum(h * B);
+          if (lane < ns) {
+            sp[W_ * SP_STRIDE + lane] = w;
+            sp[WS0_ * SP_STRIDE + lane] = w0;
+            sp[WS1_ * SP_STRIDE + lane] = w1;
+            sp[A_ * SP_STRIDE + lane] = A;
+            sp[B_ * SP_STRIDE + lane] = B;
+          }
+          const double iRC = 1. / (rho * cp);
+          if (lane < nm) sp[PR_ * SP_STRIDE + 1 + lane] = iR * (w0 - iR * w);
+          if (lane == 0) {
+            sp[PR_ * SP_STRIDE] = -iRC * HS0 - iR * rhs0;
+            sc[S_RHS0] = rhs0; sc[S_HS0] = HS0; sc[S_HS1] = HS1; sc[S_WCP] = WCP; sc[S_HA] = HA; sc[S_HB] = HB;
+          }
+        }
+      }
+    }
+    if (JAC) {
+      __syncthreads();
+      // ---------------- P3a: per-column sums (lane = (column, state)): temperature row and diagonal -------
+      for (int it = threadIdx.x; it < S * nm; it += THREADS) {
+        const int j = it / S, s = it - j * S;
+        if (first + s >= a.n) continue;
+        double *sp = lds + s * stride;
+        const double *rq = sp + N_SPEC_ARR * SP_STRIDE + N_SCAL;
+        const double *v = rq + 6 * nr;
+        double SH = 0., DG = 0.;
+        for (int t = mv.cg_off[j]; t < mv.cg_off[j + 1]; ++t) {
+          const double vt = v[mv.cg_slot[t]];
+          SH += vt * rq[5 * nr + mv.cg_r[t]];
+          DG += vt * mv.cg_dcoef[t];
+        }
+        sp[SH_ * SP_STRIDE + j] = SH;
+        sp[DG_ * SP_STRIDE + j] = DG;
+      }
+      __syncthreads();
+
+      // ---------------- P3b: Jacobian entries (lane = entry, column-major: contiguous stores) -------------
+      int nact = 0;
+      double iR[S], iC[S], iRC[S], roT[S], rhs0[S], rmw[S], HA[S], HB[S], T00[S];
+      for (int s = 0; s < S; ++s) {
+        if (first + s < a.n) nact = s + 1;
+        const double *sc = lds + s * stride + N_SPEC_ARR * SP_STRIDE;
+        iR[s] = 1. / sc[S_RHO];
+        iC[s] = 1. / sc[S_CP];
+        iRC[s] = 1. / (sc[S_RHO] * sc[S_CP]);
+        roT[s] = sc[S_RHO] / sc[S_T];
+        rhs0[s] = sc[S_RHS0];
+        rmw[s] = -sc[S_RHO] * sc[S_MMW];
+        HA[s] = sc[S_HA];
+        HB[s] = sc[S_HB];
+        T00[s] = -iRC[s] * (sc[S_HS1] + sc[S_WCP]) - rhs0[s] * sc[S_CPST] * iC[s];
+      }
+      if (a.jac) {
+        const int nent = ns * ns;
+        for (int e = threadIdx.x; e < nent; e += THREADS) {
+          const int c = e / ns, rr = e - c * ns;
+          double val[S];
+          if (c == 0) {
+            for (int s = 0; s < S; ++s) {
+              const double *sp = lds + s * stride;
+              const double *pr = sp + PR_ * SP_STRIDE;
+              val[s] = (rr == 0) ? T00[s] - roT[s] * pr[0]
+                                 : sp[WS1_ * SP_STRIDE + rr - 1] * iR[s] - roT[s] * pr[rr];
+            }
+          } else {
+            const int j = c - 1;
+            const double dj = mv.dinvm[j];
+            if (rr == 0) {
+              for (int s = 0; s < S; ++s) {
+                const double *sp = lds + s * stride;
+                val[s] = -iRC[s] * (sp[SH_ * SP_STRIDE + j] + dj * HA[s] + HB[s]) -
+                         rhs0[s] * (sp[CPI_ * SP_STRIDE + j] - sp[CPI_ * SP_STRIDE + nm]) * iC[s] +
+                         rmw[s] * dj * sp[PR_ * SP_STRIDE];
+              }
+            } else {
+              const int ii = rr - 1;
+              const int ee = j * ns + ii;
+              double acc[S];
+              for (int s = 0; s < S; ++s) acc[s] = (ii == j) ? lds[s * stride + DG_ * SP_STRIDE + j] : 0.;
+              const int t0 = mv.ex_off[ee], t1 = mv.ex_off[ee + 1];
+              for (int t = t0; t < t1; ++t) {
+                const int slot = mv.ex_slot[t];
+                const double coef = mv.ex_coef[t];
+                for (int s = 0; s < S; ++s)
+                  acc[s] += coef * lds[s * stride + N_SPEC_ARR * SP_STRIDE + N_SCAL + 6 * nr + slot];
+              }
+              for (int s = 0; s < S; ++s) {
+                const double *sp = lds + s * stride;
+                val[s] = iR[s] * (acc[s] + dj * sp[A_ * SP_STRIDE + ii] + sp[B_ * SP_STRIDE + ii]) +
+                         rmw[s] * dj * sp[PR_ * SP_STRIDE + rr];
+              }
+            }
+          }
+          for (int s = 0; s < nact; ++s) a.jac[(first + s) * a.js_i + e] = val[s];
+        }
+      }
+      if (a.sens) {
+        // raw option-0 sensitivities, (ns+1)^2 column-major with stride ns+1 (rates_sensitivities_exact.cpp:1011-1026)
+        for (int s = 0; s < nact; ++s) {
+          const double *sp = lds + s * stride;
+          const double *v = sp + N_SPEC_ARR * SP_STRIDE + N_SCAL + 6 * nr;
+          double *so = a.sens + (first + s) * (long)(ns + 1) * (ns + 1);
+          for (int e = threadIdx.x; e < (ns + 1) * (ns + 1); e += THREADS) {
+            const int c = e / (ns + 1), k = e - c * (ns + 1);
+            double val = 0.;
+            if (k < ns) {
+              if (c == 0) val = sp[WS0_ * SP_STRIDE + k];
+              else if (c == 1) val = sp[WS1_ * SP_STRIDE + k];
+              else {
+                const int j = c - 2;
+                const int ee = j * ns + k;
+                double spv = 0.;
+                for (int t = mv.eg_off[ee]; t < mv.eg_off[ee + 1]; ++t) spv += mv.eg_coef[t] * v[mv.eg_slot[t]];
+                val = spv + mv.dinvm[j] * sp[A_ * SP_STRIDE + k] + sp[B_ * SP_STRIDE + k];
+              }
+            }
+            so[e] = val;
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next group reuses the LDS
   }
 }
 
@@ -497,8 +545,14 @@ int launch_cfg(const gf_mech *m, const ChemArgs &a, hipStream_t st) {
       return fail(GF_ERR_UNSUPPORTED, "mechanism needs more LDS per workgroup than the device allows");
   }
   const long groups = (a.n + S - 1) / S;
-  if (groups > 0x7fffffffL) return fail(GF_ERR_ARG, "batch too large");
-  hipLaunchKernelGGL(kern, dim3((unsigned)groups), dim3(THREADS), lds, st, m->view, a);
+  // persistent launch: as many workgroups as fit the device at once (LDS-limited), each owning a
+  // contiguous range of groups
+  const long per_cu = std::max<long>(1, (long)std::max<size_t>(m->lds_max, 65536) / (long)std::max<size_t>(lds, 1));
+  const long width = std::max<long>(1, (long)std::max(m->n_cu, 1) * std::min<long>(per_cu, 8));
+  const long nwg = std::min(groups, width);
+  const long gpw = (groups + nwg - 1) / nwg;
+  const long grid = (groups + gpw - 1) / gpw;
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(THREADS), lds, st, m->view, a, gpw);
   if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, "chemistry kernel launch failed");
   return GF_OK;
 }
@@ -511,24 +565,28 @@ int launch_chem(const gf_mech *m, const ChemIO &a, hipStream_t st) {
   if (a.n <= 0) return GF_OK;
   static int cfg = -1;
   if (cfg < 0) {
-    cfg = 1;
+    cfg = 2;
     if (const char *e = getenv("GF_CHEM_CFG")) {
-      if (!strcmp(e, "1x64")) cfg = 0;
-      else if (!strcmp(e, "1x128")) cfg = 1;
-      else if (!strcmp(e, "1x256")) cfg = 2;
-      else if (!strcmp(e, "2x128")) cfg = 3;
-      else if (!strcmp(e, "2x256")) cfg = 4;
-      else if (!strcmp(e, "4x256")) cfg = 5;
+      if (!strcmp(e, "1x128")) cfg = 0;
+      else if (!strcmp(e, "2x256")) cfg = 1;
+      else if (!strcmp(e, "4x512")) cfg = 2;
+      else if (!strcmp(e, "4x256")) cfg = 3;
+      else if (!strcmp(e, "2x512")) cfg = 4;
     }
   }
+  int c = cfg;
+  // fewer states per workgroup when S states do not fit the workgroup LDS limit of this device
+  const size_t lim = m->lds_max;
+  if ((c == 2 || c == 3) && m->lds_per_state * 4 > lim) c = 1;
+  if ((c == 1 || c == 4) && m->lds_per_state * 2 > lim) c = 0;
+  if (m->lds_per_state > lim) return fail(GF_ERR_UNSUPPORTED, "mechanism too large for one workgroup's LDS");
   const bool J = a.jac || a.sens;
-  switch (cfg) {
-    case 0: return J ? launch_cfg<1, 64, true>(m, a, st) : launch_cfg<1, 64, false>(m, a, st);
-    case 2: return J ? launch_cfg<1, 256, true>(m, a, st) : launch_cfg<1, 256, false>(m, a, st);
-    case 3: return J ? launch_cfg<2, 128, true>(m, a, st) : launch_cfg<2, 128, false>(m, a, st);
-    case 4: return J ? launch_cfg<2, 256, true>(m, a, st) : launch_cfg<2, 256, false>(m, a, st);
-    case 5: return J ? launch_cfg<4, 256, true>(m, a, st) : launch_cfg<4, 256, false>(m, a, st);
-    default: return J ? launch_cfg<1, 128, true>(m, a, st) : launch_cfg<1, 128, false>(m, a, st);
+  switch (c) {
+    case 0: return J ? launch_cfg<1, 128, true>(m, a, st) : launch_cfg<1, 128, false>(m, a, st);
+    case 1: return J ? launch_cfg<2, 256, true>(m, a, st) : launch_cfg<2, 256, false>(m, a, st);
+    case 3: return J ? launch_cfg<4, 256, true>(m, a, st) : launch_cfg<4, 256, false>(m, a, st);
+    case 4: return J ? launch_cfg<2, 512, true>(m, a, st) : launch_cfg<2, 512, false>(m, a, st);
+    default: return J ? launch_cfg<4, 512, true>(m, a, st) : launch_cfg<4, 512, false>(m, a, st);
   }
 }
 

@@ -67,10 +67,16 @@ struct MechView {
   const int *eg_off;        // [(ns-1)*ns+1], entry index e = j*ns + i
   const int *eg_slot;
   const double *eg_coef;
-  // temperature row, column j <- slots with species j (weight: the reaction's enthalpy change)
+  // the same without the diagonal terms (i == j), which are summed per column with cg_dcoef
+  const int *ex_off;
+  const int *ex_slot;
+  const double *ex_coef;
+  // column j <- slots with species j: the temperature row (weight: the reaction's enthalpy change H_r)
+  // and the diagonal entry (weight cg_dcoef: -net_stoich_j * mw_j, or 0 if j is not a net species)
   const int *cg_off;        // [ns]
   const int *cg_slot;
   const int *cg_r;
+  const double *cg_dcoef;
 };
 
 struct Species {
@@ -106,6 +112,8 @@ struct gf_mech {
   void *dblob = nullptr;          // device copy
   gfa::MechView view;             // device pointers
   size_t lds_per_state = 0;       // bytes of LDS the chemistry kernels need per state
+  int n_cu = 0;                   // compute units of the device (persistent launch width)
+  size_t lds_max = 65536;         // largest LDS allocation one workgroup may use on this device
   // staging buffers for the host-pointer drop-ins
   void *stage = nullptr;
   size_t stage_bytes = 0;

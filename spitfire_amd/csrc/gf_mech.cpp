@@ -164,29 +164,43 @@ int finalize(gf_mech *m) {
     }
   }
   const int nm = ns - 1;
-  std::vector<int> eg_off(nm * ns + 1, 0), eg_slot, cg_off(nm + 1, 0), cg_slot, cg_r;
-  std::vector<double> eg_coef;
+  // eg: all terms of sensitivity entry (i, j); ex: the same without the diagonal terms i == j, which the
+  // kernels sum per column together with the temperature row (cg lists, cg_dcoef).
+  std::vector<int> eg_off(nm * ns + 1, 0), eg_slot, ex_off(nm * ns + 1, 0), ex_slot, cg_off(nm + 1, 0), cg_slot, cg_r;
+  std::vector<double> eg_coef, ex_coef, cg_dcoef;
   {
     std::vector<std::vector<std::pair<int, double>>> E(nm * ns);
     std::vector<std::vector<std::pair<int, int>>> C(nm);
+    std::vector<std::vector<double>> CD(nm);
     for (int r = 0; r < nr; ++r) {
       const DevRxn &d = drx[r];
       for (int p = 0; p < d.n_slot; ++p) {
         const int j = slot_sp[d.slot_off + p];
-        C[j].push_back({d.slot_off + p, r});
+        double dco = 0.;
         for (int q = 0; q < d.n_net; ++q) {
           const int i = d.net_idx[q];
           E[j * ns + i].push_back({d.slot_off + p, -d.net_st[q] * mw[i]});
+          if (i == j) dco = -d.net_st[q] * mw[i];
         }
+        C[j].push_back({d.slot_off + p, r});
+        CD[j].push_back(dco);
       }
     }
     for (int e = 0; e < nm * ns; ++e) {
       eg_off[e + 1] = eg_off[e] + (int)E[e].size();
       for (auto &t : E[e]) { eg_slot.push_back(t.first); eg_coef.push_back(t.second); }
+      const bool diag = (e / ns) == (e % ns);
+      ex_off[e + 1] = ex_off[e] + (diag ? 0 : (int)E[e].size());
+      if (!diag)
+        for (auto &t : E[e]) { ex_slot.push_back(t.first); ex_coef.push_back(t.second); }
     }
     for (int j = 0; j < nm; ++j) {
       cg_off[j + 1] = cg_off[j] + (int)C[j].size();
-      for (auto &t : C[j]) { cg_slot.push_back(t.first); cg_r.push_back(t.second); }
+      for (size_t q = 0; q < C[j].size(); ++q) {
+        cg_slot.push_back(C[j][q].first);
+        cg_r.push_back(C[j][q].second);
+        cg_dcoef.push_back(CD[j][q]);
+      }
     }
   }
 
@@ -217,12 +231,16 @@ int finalize(gf_mech *m) {
   v.eg_off = (const int *)P(a.put(eg_off));
   v.eg_slot = (const int *)P(a.put(eg_slot));
   v.eg_coef = (const double *)P(a.put(eg_coef));
+  v.ex_off = (const int *)P(a.put(ex_off));
+  v.ex_slot = (const int *)P(a.put(ex_slot));
+  v.ex_coef = (const double *)P(a.put(ex_coef));
   v.cg_off = (const int *)P(a.put(cg_off));
   v.cg_slot = (const int *)P(a.put(cg_slot));
   v.cg_r = (const int *)P(a.put(cg_r));
+  v.cg_dcoef = (const double *)P(a.put(cg_dcoef));
   m->blob.swap(a.buf);
   m->drx = drx;
-  m->lds_per_state = sizeof(double) * (11 * NS_MAX + 16 + 6 * (size_t)nr + (size_t)nslot);
+  m->lds_per_state = sizeof(double) * (13 * NS_MAX + 16 + 6 * (size_t)nr + (size_t)nslot);
   m->finalized = true;
   return GF_OK;
 }
@@ -250,6 +268,12 @@ int ensure_device(gf_mech *m) {
   if (hipMemcpy(m->dblob, m->blob.data(), m->blob.size(), hipMemcpyHostToDevice) != hipSuccess)
     return fail(GF_ERR_HIP, "upload of mechanism tables failed");
   m->device = dev;
+  int ncu = 0, lmax = 0, lopt = 0;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  (void)hipDeviceGetAttribute(&lopt, hipDeviceAttributeSharedMemPerBlockOptin, dev);
+  m->n_cu = ncu > 0 ? ncu : 1;
+  m->lds_max = std::max(lmax, lopt) > 0 ? (size_t)std::max(lmax, lopt) : 65536;
   const char *b = static_cast<const char *>(m->dblob);
   MechView v = m->view_host;
   v.spc_c = rebase(v.spc_c, b); v.invmw = rebase(v.invmw, b); v.mw = rebase(v.mw, b); v.dinvm = rebase(v.dinvm, b);
@@ -258,7 +282,9 @@ int ensure_device(gf_mech *m) {
   v.sp_idx = rebase(v.sp_idx, b); v.sp_ord = rebase(v.sp_ord, b); v.sp_invmw = rebase(v.sp_invmw, b);
   v.sg_off = rebase(v.sg_off, b); v.sg_r = rebase(v.sg_r, b); v.sg_coef = rebase(v.sg_coef, b);
   v.eg_off = rebase(v.eg_off, b); v.eg_slot = rebase(v.eg_slot, b); v.eg_coef = rebase(v.eg_coef, b);
+  v.ex_off = rebase(v.ex_off, b); v.ex_slot = rebase(v.ex_slot, b); v.ex_coef = rebase(v.ex_coef, b);
   v.cg_off = rebase(v.cg_off, b); v.cg_slot = rebase(v.cg_slot, b); v.cg_r = rebase(v.cg_r, b);
+  v.cg_dcoef = rebase(v.cg_dcoef, b);
   m->view = v;
   if (!m->stream) {
     if (hipStreamCreate(&m->stream) != hipSuccess) return fail(GF_ERR_HIP, "hipStreamCreate failed");

@@ -1,0 +1,389 @@
+"""Steady nonpremixed flamelets: FlameletSpec / Flamelet mirroring src/spitfire/chemistry/flamelet.py.
+
+Grid (flamelet.py:283-341), dissipation rate (240-266), initial conditions (486-597), the griffon RHS /
+Jacobian wrappers (642-762), block-Thomas setup and solve (756-762, 911-915), steady_solve_newton
+(1352-1485), steady_solve_psitc (1487-1703) and compute_steady_state (1705-1778), running on the MI355X
+griffon drop-in (spitfire_amd.griffon).
+
+Differences from the reference, all documented here:
+  * streams are spitfire_amd.mechanism.Stream objects (no Cantera); the 'equilibrium' initial condition
+    is not available (it needs Cantera's equilibrate);
+  * pseudo-transient continuation takes the eigenvalue estimate `expeig` (flamelet.py:1597-1604, LAPACK
+    dgeev per block in the reference) from `eigenvalue_bound`, default 0: the local pseudo-time step then
+    follows the reference's ramp ds <- min(ds_ramp * ds, ds_max) without the explosive-mode limiter. The
+    converged steady state does not depend on ds, only the iteration path does;
+  * compute_steady_state falls back from Newton to psitc; the ESDIRK fallback is not built.
+"""
+import numpy as np
+from scipy.special import erfinv
+
+
+def compute_dissipation_rate(mixture_fraction, max_dissipation_rate, form='Peters'):
+    """chi(Z) = chi_max exp(-2 erfinv(2Z-1)^2) (Peters), or constant (flamelet.py:240-266)."""
+    if form in ('Peters', 'peters'):
+        return max_dissipation_rate * np.exp(-2. * (erfinv(2. * mixture_fraction - 1.)) ** 2)
+    x = np.zeros_like(mixture_fraction)
+    x[:] = max_dissipation_rate
+    return x
+
+
+def uniform_grid(grid_points):
+    z = np.linspace(0., 1., grid_points)
+    return z, z[1:] - z[:-1]
+
+
+def clustered_grid(grid_points, grid_cluster_point, grid_cluster_intensity=6.):
+    """Anderson's sinh clustering about a point (flamelet.py:300-341)."""
+    if grid_cluster_intensity < 1.e-16:
+        raise ValueError('cluster_coeff must be strictly positive! Given value: ' + str(grid_cluster_intensity))
+    if grid_cluster_point < 0. or grid_cluster_point > 1.:
+        raise ValueError('z_cluster must be between 0 and 1! Given value: ' + str(grid_cluster_point))
+    b = grid_cluster_intensity
+    z = np.linspace(0., 1., grid_points)
+    zo = 1.0 / (2.0 * b) * np.log((1. + (np.exp(b) - 1.) * grid_cluster_point) /
+                                  (1. + (np.exp(-b) - 1.) * grid_cluster_point))
+    a = np.sinh(b * zo)
+    for i in range(grid_points):
+        z[i] = grid_cluster_point / a * (np.sinh(b * (z[i] - zo)) + a)
+    z[-1] = 1.
+    return z, z[1:] - z[:-1]
+
+
+class FlameletSpec:
+    """Flamelet specification (flamelet.py:35-237), adiabatic flamelets."""
+
+    def __init__(self, mech_spec=None, initial_condition='Burke-Schumann', oxy_stream=None, fuel_stream=None,
+                 max_dissipation_rate=None, stoich_dissipation_rate=None, dissipation_rate_form='Peters',
+                 dissipation_rate=None, grid=None, grid_points=None, grid_type='clustered',
+                 grid_cluster_intensity=4., grid_cluster_point='stoichiometric', heat_transfer='adiabatic',
+                 include_enthalpy_flux=False, include_variable_cp=False, rates_sensitivity_type='dense',
+                 sensitivity_transform_type='exact'):
+        if heat_transfer != 'adiabatic':
+            raise NotImplementedError('nonadiabatic flamelets are not supported by this implementation')
+        self.mech_spec = mech_spec
+        self.initial_condition = initial_condition
+        self.oxy_stream = oxy_stream
+        self.fuel_stream = fuel_stream
+        self.max_dissipation_rate = max_dissipation_rate
+        self.stoich_dissipation_rate = stoich_dissipation_rate
+        self.dissipation_rate_form = dissipation_rate_form
+        self.dissipation_rate = dissipation_rate
+        self.grid = grid
+        self.grid_points = grid_points
+        self.grid_type = grid_type
+        self.grid_cluster_intensity = grid_cluster_intensity
+        self.grid_cluster_point = grid_cluster_point
+        self.heat_transfer = heat_transfer
+        self.include_enthalpy_flux = include_enthalpy_flux
+        self.include_variable_cp = include_variable_cp
+        self.rates_sensitivity_type = rates_sensitivity_type
+        self.sensitivity_transform_type = sensitivity_transform_type
+
+
+class Flamelet:
+    """An adiabatic steady flamelet solved with the griffon drop-in (flamelet.py:269-1778)."""
+
+    def __init__(self, flamelet_specs=None, **kwargs):
+        fs = flamelet_specs if isinstance(flamelet_specs, FlameletSpec) else FlameletSpec(**(flamelet_specs or kwargs))
+        self._mechanism = fs.mech_spec
+        self._oxy_stream, self._fuel_stream = fs.oxy_stream, fs.fuel_stream
+        if abs(self._fuel_stream.P - self._oxy_stream.P) / self._oxy_stream.P > 1.e-6:
+            raise ValueError('The pressure of the fuel and oxidizer streams must be the same.')
+        self._pressure = float(self._oxy_stream.P)
+        self._n_species = self._mechanism.n_species
+        self._n_equations = self._n_species
+        self._state_fuel = self._fuel_stream.state
+        self._state_oxy = self._oxy_stream.state
+        mech = self._mechanism
+        if fs.grid is not None:
+            self._z = np.array(fs.grid, dtype=np.float64)
+            self._dz = self._z[1:] - self._z[:-1]
+        elif fs.grid_type == 'uniform':
+            self._z, self._dz = uniform_grid(fs.grid_points)
+        else:
+            zc = fs.grid_cluster_point
+            if zc == 'stoichiometric':
+                zc = mech.stoich_mixture_fraction(self._fuel_stream, self._oxy_stream)
+            self._z, self._dz = clustered_grid(fs.grid_points, zc, fs.grid_cluster_intensity)
+        self._nz_interior = self._z.size - 2
+        self._n_dof = self._n_equations * self._nz_interior
+        if fs.dissipation_rate is not None:
+            self._x = np.array(fs.dissipation_rate, dtype=np.float64)
+            self._max_dissipation_rate = np.max(self._x)
+        else:
+            if fs.max_dissipation_rate is not None:
+                self._max_dissipation_rate = fs.max_dissipation_rate
+            elif fs.dissipation_rate_form in ('peters', 'Peters'):
+                zst = mech.stoich_mixture_fraction(self._fuel_stream, self._oxy_stream)
+                self._max_dissipation_rate = fs.stoich_dissipation_rate / np.exp(-2. * (erfinv(2. * zst - 1.)) ** 2)
+            else:
+                self._max_dissipation_rate = fs.stoich_dissipation_rate
+            self._x = compute_dissipation_rate(self._z, self._max_dissipation_rate, fs.dissipation_rate_form)
+        self._lewis_numbers = np.ones(self._n_species)
+        self._include_enthalpy_flux = fs.include_enthalpy_flux
+        self._include_variable_cp = fs.include_variable_cp
+        self._rsopt = {'dense': 0, 'no-TBAF': 1, 'sparse': 2}[fs.rates_sensitivity_type]
+        self._stopt = {'exact': 0}[fs.sensitivity_transform_type]
+        self._initial_state = self._make_initial_state(fs.initial_condition)
+        self._current_state = np.copy(self._initial_state)
+        self._variable_scales = np.ones(self._n_dof)
+        self._variable_scales[::self._n_equations] = 1.e3
+        self._griffon = mech.griffon
+        n, nzi = self._n_species, self._nz_interior
+        self._maj_coeff_griffon = np.zeros(self._n_dof)
+        self._sub_coeff_griffon = np.zeros(self._n_dof)
+        self._sup_coeff_griffon = np.zeros(self._n_dof)
+        self._mcoeff_griffon = np.zeros(nzi)
+        self._ncoeff_griffon = np.zeros(nzi)
+        self._setup_stencils()
+        self._jac_nelements_griffon = int(n * (nzi * n + 2 * (nzi - 1)))
+        self._block_thomas_l_values = np.zeros(n * n * nzi)
+        self._block_thomas_d_pivots = np.zeros(self._jac_nelements_griffon, dtype=np.int32)
+        self._iteration_count = None
+
+    def _setup_stencils(self):
+        self._griffon.flamelet_stencils(self._dz, self._nz_interior, self._x, 1. / self._lewis_numbers,
+                                        self._maj_coeff_griffon, self._sub_coeff_griffon, self._sup_coeff_griffon,
+                                        self._mcoeff_griffon, self._ncoeff_griffon)
+
+    # ---------------- initial conditions (flamelet.py:486-597) ----------------
+    def _make_initial_state(self, ic):
+        mech, nzi, neq = self._mechanism, self._nz_interior, self._n_equations
+        if isinstance(ic, np.ndarray):
+            if ic.size != self._n_dof:
+                raise ValueError('size of initial condition is incorrect!')
+            return np.copy(ic)
+        st = np.ndarray((nzi, neq))
+        zi = self._z[1:-1]
+        if ic == 'linear-TY':
+            for i in range(neq):
+                st[:, i] = self._state_oxy[i] + (self._state_fuel[i] - self._state_oxy[i]) * zi
+        elif ic == 'unreacted':
+            for i in range(nzi):
+                m = mech.mix_streams([(self._oxy_stream, 1 - zi[i]), (self._fuel_stream, zi[i])], 'mass', 'HP')
+                st[i, :] = m.state
+        elif ic == 'Burke-Schumann':
+            zst = mech.stoich_mixture_fraction(self._fuel_stream, self._oxy_stream)
+            stmix = mech.mix_streams([(self._oxy_stream, 1 - zst), (self._fuel_stream, zst)], 'mass', 'HP')
+            stat = mech._get_atoms_in_stream(stmix, [a for a in ('H', 'C', 'O', 'N') if a in
+                                                     mech.mech_data['elements']] )
+            comp = 'N2: %.17g H2O: %.17g' % (stat.get('N', 0.) / 2, 0.5 * stat.get('H', 0.))
+            if 'CO2' in mech.species_names:
+                comp += ' CO2: %.17g' % stat.get('C', 0.)
+            sw = mech.stream('HPX', (stmix.H, self._pressure, comp))
+            Yst, Yo, Yf = sw.Y, self._oxy_stream.Y, self._fuel_stream.Y
+            for i in range(nzi):
+                z = zi[i]
+                h = mech.mix_streams([(self._oxy_stream, 1 - z), (self._fuel_stream, z)], 'mass', 'HP').H
+                Y = Yo + z / zst * (Yst - Yo) if z <= zst else Yst + (z - zst) / (1. - zst) * (Yf - Yst)
+                st[i, :] = mech.stream('HPY', (h, self._pressure, Y)).state
+        else:
+            raise ValueError('unsupported initial condition ' + str(ic))
+        return st.ravel().copy()
+
+    # ---------------- griffon wrappers (flamelet.py:642-762, 911-915) ----------------
+    def _adiabatic_rhs(self, t, state_interior):
+        rhs = np.zeros(self._n_dof)
+        null = np.zeros(1)
+        self._griffon.flamelet_rhs(state_interior, self._pressure, self._state_oxy, self._state_fuel, True, null, null,
+                                   null, null, self._nz_interior, self._maj_coeff_griffon, self._sub_coeff_griffon,
+                                   self._sup_coeff_griffon, self._mcoeff_griffon, self._ncoeff_griffon, self._x,
+                                   self._include_enthalpy_flux, self._include_variable_cp, False, rhs)
+        return rhs
+
+    def _adiabatic_jac(self, state_interior, scale_and_offset=False, prefactor=0.):
+        values = np.zeros(self._jac_nelements_griffon)
+        null = np.zeros(1)
+        self._griffon.flamelet_jacobian(state_interior, self._pressure, self._state_oxy, self._state_fuel, True, null,
+                                        null, null, null, self._nz_interior, self._maj_coeff_griffon,
+                                        self._sub_coeff_griffon, self._sup_coeff_griffon, self._mcoeff_griffon,
+                                        self._ncoeff_griffon, self._x, False, 0., scale_and_offset, prefactor,
+                                        self._rsopt, self._stopt, self._include_enthalpy_flux,
+                                        self._include_variable_cp, False, null, values)
+        return values
+
+    def rhs(self, t, state_interior):
+        return self._adiabatic_rhs(t, state_interior)
+
+    def jac(self, state_interior):
+        return self._adiabatic_jac(state_interior)
+
+    def eigenvalue_bound(self, state_interior, diffterm):
+        """Substitute for the reference's per-block dgeev estimate (see module docstring)."""
+        return np.zeros(self._n_dof)
+
+    # ---------------- linear algebra (griffon py_btddod_*) ----------------
+    def _factorize(self, values):
+        from .griffon import py_btddod_full_factorize
+        py_btddod_full_factorize(values, self._nz_interior, self._n_equations, self._block_thomas_l_values,
+                                 self._block_thomas_d_pivots)
+
+    def _solve(self, values, rhs):
+        from .griffon import py_btddod_full_solve
+        out = np.zeros(self._n_dof)
+        py_btddod_full_solve(values, self._block_thomas_l_values, self._block_thomas_d_pivots, rhs,
+                             self._nz_interior, self._n_equations, out)
+        return out
+
+    def _scale_add_diag(self, values, a, diag, b):
+        from .griffon import py_btddod_scale_and_add_diagonal
+        py_btddod_scale_and_add_diagonal(values, a, diag, b, self._nz_interior, self._n_equations)
+
+    # ---------------- steady solvers (flamelet.py:1352-1778) ----------------
+    def steady_solve_newton(self, initial_guess=None, tolerance=1.e-6, max_iterations=10, max_factor_line_search=1.5,
+                            max_allowed_residual=1.e6, min_allowable_state_var=-1.e-6, norm_order=np.inf,
+                            log_rate=100000, verbose=True):
+        state = np.copy(self._initial_state if initial_guess is None else initial_guess)
+        inv_dofscales = 1. / self._variable_scales
+        iteration_count = 0
+        res = tolerance + 1.
+        rhs = self.rhs(0., state)
+        evaluate_jacobian = True
+        J = None
+        while res > tolerance and iteration_count < max_iterations:
+            iteration_count += 1
+            if evaluate_jacobian:
+                J = -self.jac(state)
+                self._factorize(J)
+                evaluate_jacobian = False
+            dstate = self._solve(J, rhs)
+            if np.any(~np.isfinite(dstate)):
+                return None, None, False
+            norm_rhs_old = np.linalg.norm(rhs * inv_dofscales, ord=norm_order)
+            alpha = 1.
+            rhs = self.rhs(0., state + dstate)
+            if np.any(~np.isfinite(rhs)):
+                return None, None, False
+            while np.linalg.norm(rhs * inv_dofscales, ord=norm_order) > max_factor_line_search * norm_rhs_old \
+                    and alpha > 0.001:
+                alpha *= 0.5
+                dstate *= alpha  # as in the reference, the step shrinks by the running product of alphas
+                rhs = self.rhs(0., state + dstate)
+                evaluate_jacobian = True
+            state += dstate
+            res = np.linalg.norm(rhs * inv_dofscales, ord=norm_order)
+            if res > max_allowed_residual or np.min(state) < min_allowable_state_var:
+                return None, None, False
+        state[state < 0] = 0.
+        if res > tolerance:
+            return None, None, False
+        self._current_state = np.copy(state)
+        self._iteration_count = iteration_count
+        return state, iteration_count, True
+
+    def steady_solve_psitc(self, initial_guess=None, tolerance=1.e-6, max_iterations=400,
+                           min_allowable_state_var=-1.e-6, ds_init=1., ds_init_decrease=4., adaptive_restart=True,
+                           diffusion_factor=4., global_ds=False, ds_safety=0.1, ds_ramp=1.1, ds_max=1.e4,
+                           max_factor_line_search=1.5, max_allowed_residual=1.e6, log_rate=100000,
+                           norm_order=np.inf, _recursion_depth=0, max_recursion_depth=20, verbose=True):
+        args = dict(locals())
+        del args['self']
+
+        def restart():
+            if _recursion_depth > max_recursion_depth or not adaptive_restart:
+                return None, None, False, np.min(ds)
+            args['ds_init'] = ds_init / ds_init_decrease
+            args['_recursion_depth'] = _recursion_depth + 1
+            return self.steady_solve_psitc(**args)
+
+        state = np.copy(self._initial_state if initial_guess is None else initial_guess)
+        inv_dofscales = 1. / self._variable_scales
+        diffterm = diffusion_factor * self._max_dissipation_rate
+        ds = np.full(self._n_dof, float(ds_init))
+        iteration_count = 0
+        jac_age = 0
+        res = tolerance + 1.
+        rhs = self.rhs(0., state)
+        evaluate_jacobian = True
+        J = None
+        while res > tolerance and iteration_count < max_iterations:
+            iteration_count += 1
+            if evaluate_jacobian:
+                J = self.jac(state)
+                expeig = self.eigenvalue_bound(state, diffterm)
+                ds = np.min(np.vstack([ds_safety / (expeig + 1.e-16), ds_ramp * ds, ds_max * np.ones(self._n_dof)]),
+                            axis=0)
+                if global_ds or iteration_count == 1:
+                    ds[:] = np.min(ds)
+                self._scale_add_diag(J, -1., 1. / ds, 1.)
+                self._factorize(J)
+                jac_age = 0
+            else:
+                jac_age += 1
+            evaluate_jacobian = (jac_age == 8) or res > 1.e-2
+            dstate = self._solve(J, rhs)
+            if np.any(~np.isfinite(dstate)):
+                return restart()
+            norm_rhs_old = np.linalg.norm(rhs * inv_dofscales, ord=norm_order)
+            alpha = 1.
+            rhs = self.rhs(0., state + dstate)
+            if np.any(~np.isfinite(rhs)):
+                return restart()
+            while np.linalg.norm(rhs * inv_dofscales, ord=norm_order) > max_factor_line_search * norm_rhs_old \
+                    and alpha > 0.001:
+                alpha *= 0.5
+                dstate *= alpha
+                rhs = self.rhs(0., state + dstate)
+                evaluate_jacobian = True
+            state += dstate
+            res = np.linalg.norm(rhs * inv_dofscales, ord=norm_order)
+            if res > max_allowed_residual or np.min(state) < min_allowable_state_var:
+                return restart()
+        state[state < 0] = 0.
+        if res > tolerance:
+            return None, None, False, np.min(ds)
+        self._iteration_count = iteration_count
+        self._current_state = np.copy(state)
+        return state, iteration_count, True, np.min(ds)
+
+    def compute_steady_state(self, tolerance=1.e-6, verbose=False, use_psitc=True, newton_args=None,
+                             psitc_args=None):
+        nargs = {'tolerance': tolerance, 'verbose': verbose, 'max_iterations': 38}
+        nargs.update(newton_args or {})
+        state, it, conv = self.steady_solve_newton(**nargs)
+        if conv:
+            return state
+        if use_psitc:
+            pargs = {'tolerance': tolerance, 'verbose': verbose, 'max_iterations': 400}
+            pargs.update(psitc_args or {})
+            state, it, conv, _ = self.steady_solve_psitc(**pargs)
+            if conv:
+                return state
+        raise RuntimeError('steady flamelet solve did not converge')
+
+    # ---------------- accessors ----------------
+    @property
+    def mixture_fraction_grid(self):
+        return self._z.copy()
+
+    @property
+    def dissipation_rate(self):
+        return self._x.copy()
+
+    @property
+    def initial_interior_state(self):
+        return self._initial_state.copy()
+
+    @property
+    def current_interior_state(self):
+        return self._current_state.copy()
+
+    @property
+    def iteration_count(self):
+        return self._iteration_count
+
+    @property
+    def n_interior_points(self):
+        return self._nz_interior
+
+    @property
+    def oxy_stream(self):
+        return self._oxy_stream
+
+    @property
+    def fuel_stream(self):
+        return self._fuel_stream
+
+    @property
+    def pressure(self):
+        return self._pressure

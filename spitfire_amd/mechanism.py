@@ -13,6 +13,43 @@ from .griffon import PyCombustionKernels
 from .mechdata import sorted_reactions, species_molecular_weight
 
 
+class Stream:
+    """A gas mixture at (T, P, Y): the role Cantera Quantity objects play for the reference's streams
+    (mechanism.py:602-646). Enthalpy and mixing use the mechanism's NASA7 / constant-cp data."""
+
+    def __init__(self, mech, T, P, Y):
+        self._mech = mech
+        self.T = float(T)
+        self.P = float(P)
+        Y = np.asarray(Y, dtype=np.float64)
+        self.Y = Y / np.sum(Y)
+
+    @property
+    def species_names(self):
+        return self._mech.species_names
+
+    @property
+    def mean_molecular_weight(self):
+        return 1. / np.sum(self.Y / self._mech.molecular_weights)
+
+    @property
+    def X(self):
+        return self.Y / self._mech.molecular_weights * self.mean_molecular_weight
+
+    @property
+    def H(self):
+        """mixture enthalpy per unit mass (J/kg)"""
+        return float(np.dot(self._mech.species_enthalpies(self.T), self.Y))
+
+    @property
+    def state(self):
+        """[T, Y_0 .. Y_{ns-2}], the reactor / flamelet state vector"""
+        return np.hstack([self.T, self.Y[:-1]])
+
+    def n_atoms(self, species, element):
+        return self._mech.mech_data['species'][species]['atoms'].get(element, 0.)
+
+
 class ChemicalMechanismSpec:
     def __init__(self, mech_data):
         self._mech_data = mech_data
@@ -121,6 +158,115 @@ class ChemicalMechanismSpec:
 
     def species_index(self, name):
         return self._species_index[name]
+
+    # ---------------- streams (mechanism.py:602-757) ----------------
+    _element_stoichiometry = {'O': -1.0, 'H': 0.5, 'C': 2.0, 'Al': 1.5, 'U': 1.0, 'Ar': 0.0, 'N': 0.0, 'He': 0.0}
+
+    def _composition(self, spec, basis):
+        y = np.zeros(self.n_species)
+        if isinstance(spec, str):
+            import re
+            for name, val in re.findall(r'([^\s,:]+)\s*:\s*([-+0-9.eE]+)', spec):
+                matches = [n for n in self._species_names if n.lower() == name.lower()]
+                if not matches:
+                    raise KeyError('unknown species ' + name)
+                y[self._species_index[matches[0]]] += float(val)
+        else:
+            y[:] = np.asarray(spec, dtype=np.float64)
+        if basis == 'X':
+            y = y * self._mw
+        return y / np.sum(y)
+
+    def stream(self, properties=None, values=None, stp_air=False):
+        """Build a Stream from 'TPY', 'TPX', 'HPY' or 'HPX' properties (or stp_air=True: O2:1 N2:3.74 at 300 K,
+        1 atm)."""
+        if stp_air:
+            return Stream(self, 300., 101325., self._composition('O2:1 N2:3.74', 'X'))
+        if properties not in ('TPY', 'TPX', 'HPY', 'HPX'):
+            raise ValueError('supported stream properties: TPY, TPX, HPY, HPX')
+        a, p, comp = values
+        Y = self._composition(comp, properties[2])
+        if properties[0] == 'T':
+            return Stream(self, a, p, Y)
+        return Stream(self, self.temperature_from_enthalpy(a, Y), p, Y)
+
+    def species_enthalpies(self, T):
+        """species enthalpies per unit mass at T (thermodynamics_kernels.cpp:262-351 clamping rules)"""
+        h = np.zeros(self.n_species)
+        R = self.gas_constant
+        for i, n in enumerate(self._species_names):
+            cp = self._mech_data['species'][n]['heat-capacity']
+            if cp['type'] == 'constant':
+                h[i] = (cp['h0'] + cp['cp'] * (T - cp['T0'])) / self._mw[i]
+                continue
+            Tmid, Tmin, Tmax = cp['Tmid'], cp['Tmin'], cp['Tmax']
+            Te = min(max(T, Tmin), Tmax)
+            a = np.asarray(cp['low-coeffs'] if Te <= Tmid else cp['high-coeffs'])
+            hRT = a[0] * Te + a[1] * Te ** 2 / 2 + a[2] * Te ** 3 / 3 + a[3] * Te ** 4 / 4 + a[4] * Te ** 5 / 5 + a[5]
+            cpe = a[0] + Te * (a[1] + Te * (a[2] + Te * (a[3] + Te * a[4])))
+            h[i] = R * (hRT + cpe * (T - Te)) / self._mw[i]
+        return h
+
+    def species_cp(self, T):
+        cpv = np.zeros(self.n_species)
+        R = self.gas_constant
+        for i, n in enumerate(self._species_names):
+            cp = self._mech_data['species'][n]['heat-capacity']
+            if cp['type'] == 'constant':
+                cpv[i] = cp['cp'] / self._mw[i]
+                continue
+            Te = min(max(T, cp['Tmin']), cp['Tmax'])
+            a = np.asarray(cp['low-coeffs'] if Te <= cp['Tmid'] else cp['high-coeffs'])
+            cpv[i] = R * (a[0] + Te * (a[1] + Te * (a[2] + Te * (a[3] + Te * a[4])))) / self._mw[i]
+        return cpv
+
+    def temperature_from_enthalpy(self, h, Y, T0=1000., tol=1e-12):
+        """T such that h_mix(T, Y) = h, by Newton iteration in FP64."""
+        T = T0
+        for _ in range(100):
+            f = np.dot(self.species_enthalpies(T), Y) - h
+            dT = f / np.dot(self.species_cp(T), Y)
+            T -= dT
+            if abs(dT) < tol * T:
+                break
+        return float(T)
+
+    @staticmethod
+    def mix_streams(streams, basis, constant='HP'):
+        """Mix [(stream, amount)] by mass (or moles) at constant HP or TP (mechanism.py:648-684)."""
+        mech = streams[0][0]._mech
+        m = np.array([amt if basis == 'mass' else amt * s.mean_molecular_weight for s, amt in streams])
+        Y = sum(mi * s.Y for mi, (s, _) in zip(m, streams)) / np.sum(m)
+        P = streams[0][0].P
+        if constant == 'HP':
+            H = sum(mi * s.H for mi, (s, _) in zip(m, streams)) / np.sum(m)
+            return Stream(mech, mech.temperature_from_enthalpy(H, Y, T0=streams[0][0].T), P, Y)
+        if constant == 'TP':
+            return Stream(mech, streams[0][0].T, P, Y)
+        raise ValueError('constant must be HP or TP')
+
+    def _get_atoms_in_stream(self, stream, atom_names):
+        out = {a: 0. for a in atom_names}
+        X = stream.X
+        for i, n in enumerate(self._species_names):
+            for a in atom_names:
+                out[a] += X[i] * self._mech_data['species'][n]['atoms'].get(a, 0.)
+        return out
+
+    def stoich_molar_fuel_to_oxy_ratio(self, fuel_stream, oxy_stream):
+        atoms = [a for a in self._element_stoichiometry if a in self._mech_data['elements']]
+        fa = self._get_atoms_in_stream(fuel_stream, atoms)
+        oa = self._get_atoms_in_stream(oxy_stream, atoms)
+        es = self._element_stoichiometry
+        return -sum(es[a] * oa[a] for a in atoms) / sum(es[a] * fa[a] for a in atoms)
+
+    def stoich_mass_fuel_to_oxy_ratio(self, fuel_stream, oxy_stream):
+        return (fuel_stream.mean_molecular_weight / oxy_stream.mean_molecular_weight *
+                self.stoich_molar_fuel_to_oxy_ratio(fuel_stream, oxy_stream))
+
+    def stoich_mixture_fraction(self, fuel_stream, oxy_stream):
+        eta = self.stoich_mass_fuel_to_oxy_ratio(fuel_stream, oxy_stream)
+        return eta / (1. + eta)
 
     def __getstate__(self):
         return {'mech_data': self._mech_data}

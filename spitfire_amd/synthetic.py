@@ -15,7 +15,8 @@ Rate parameters follow the workload definitions: A log-uniform in [1e6, 1e14] in
 [0, 200] kJ/mol; reverse rates come from equilibrium constants. Heat capacities are smooth,
 monotone seeded cp(T) curves, least-squares fitted by NASA7 polynomials on [300, 1000] K and
 [1000, 3500] K with cp, h and s continuous at 1000 K. Formation enthalpies follow a seeded
-bond-energy model so that H2/O2 mixtures are exothermic (H2O-like products are stable).
+bond-class model (atomization enthalpies minus H-X and heavy-heavy bond energies in textbook ranges), so that
+H2/O2 mixtures are exothermic (H2O-like products are stable).
 Every reaction conserves elements, so mass production rates sum to zero, and is written in its exothermic
 direction (recombinations as A + B (+M) -> C), so reverse rates from equilibrium constants stay bounded.
 """
@@ -127,9 +128,16 @@ def make_mechanism(n_species=53, n_reactions=325, n_three_body=None, n_troe=None
         if c in _STANDARD:
             h298 = 0.0
         else:
+            # bond-class model: H atoms bonded to heavy atoms (360-460 kJ/mol per bond, H-H 436 when there is no
+            # heavy atom), heavy-heavy single/multiple bonds 150-350 kJ/mol (textbook ranges)
             atomization = sum(c[j] * _ATOMIZATION[e] for j, e in enumerate('HON'))
-            nbonds = max(natoms - 1, 0)
-            h298 = atomization - sum(rng.uniform(380.0, 520.0) for _ in range(nbonds))
+            nh, nheavy = c[0], c[1] + c[2]
+            if nheavy == 0:
+                bonds = 436.0 * (nh // 2)
+            else:
+                bonds = sum(rng.uniform(360.0, 460.0) for _ in range(nh))
+                bonds += sum(rng.uniform(150.0, 350.0) for _ in range(nheavy - 1))
+            h298 = atomization - bonds
         h298_of.append(h298)
         s298 = 14.0 + 4.0 * (natoms - 1) + rng.uniform(-1.0, 3.0)
         low, high = _nasa7_fit(rng, natoms, h298, s298)

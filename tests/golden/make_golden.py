@@ -125,9 +125,19 @@ def flamelet_fixture(name, mech, nzi, seed, variants=((0, 0), (0, 1), (1, 0), (1
                                           prefactor=-2.5e-4)[0]
     jac_eig, expeig = ref.flamelet_jacobian(state, p, oxy, fuel, nzi, st, chi, compute_eig=True, diffterm=4. * 5.)
     out['expeig'] = expeig
-    # the pseudo-transient matrix diag(1/ds) - J (flamelet.py:1599-1606) with ds = 0.1 / (expeig + 1),
-    # built by btddod_scale_and_add_diagonal, then factored and solved by block Thomas
-    diag = (np.max(expeig) + 1.) / 0.1 * (1. + 0.5 * np.abs(np.cos(np.arange(nzi * ns))))
+    # the pseudo-transient matrix diag(1/ds) - J (flamelet.py:1599-1606), with 1/ds at 1% of the largest
+    # absolute row sum of J (a stiff flamelet Jacobian shifted by its spectral estimate alone is too badly
+    # conditioned for a 1e-10 comparison of two LU orderings), built by btddod_scale_and_add_diagonal, then
+    # factored and solved by block Thomas
+    rowsum = np.zeros(nzi * ns)
+    jv = out['jac_00']
+    for i in range(nzi):
+        blk = jv[i * ns * ns:(i + 1) * ns * ns].reshape(ns, ns)  # [col, row]
+        rowsum[i * ns:(i + 1) * ns] += np.sum(np.abs(blk), axis=0)
+    off = nzi * ns * ns
+    rowsum[ns:] += np.abs(jv[off:off + (nzi - 1) * ns])
+    rowsum[:-ns] += np.abs(jv[off + (nzi - 1) * ns:])
+    diag = 0.01 * np.max(rowsum) * (1. + 0.5 * np.abs(np.cos(np.arange(nzi * ns))))
     out['sad_diag'] = diag
     out['sad'] = ref.btddod_scale_and_add_diagonal(out['jac_00'], -1., diag, 1., nzi, ns)
     A = out['sad']
@@ -135,6 +145,13 @@ def flamelet_fixture(name, mech, nzi, seed, variants=((0, 0), (0, 1), (1, 0), (1
     b = np.sin(np.arange(nzi * ns) + 1.)
     out['btd_A'], out['btd_d'], out['btd_l'], out['btd_piv'] = A, d, l, piv
     out['btd_b'], out['btd_x'] = b, ref.btddod_full_solve(d, l, piv, b, nzi, ns)
+    # a random block-tridiagonal system whose factorization must pivot (Gaussian diagonal blocks)
+    rng = np.random.default_rng(11 + ns)
+    R = np.concatenate([rng.standard_normal(nzi * ns * ns) + np.tile(np.eye(ns).ravel(), nzi),
+                        0.3 * rng.standard_normal(2 * (nzi - 1) * ns)])
+    d, l, piv = ref.btddod_full_factorize(R, nzi, ns)
+    out['btdr_A'], out['btdr_piv'] = R, piv
+    out['btdr_x'] = ref.btddod_full_solve(d, l, piv, b, nzi, ns)
     np.savez_compressed(os.path.join(HERE, name), **out)
     print(name, nzi, 'interior points')
 

@@ -72,15 +72,18 @@ def test_flamelet_dropins_match_reference_fixtures(fx, mech, request):
     v = f['jac_00'].copy()
     py_btddod_scale_and_add_diagonal(v, -1., f['sad_diag'], 1., nzi, ns)
     assert np.max(np.abs(v - f['sad']) / np.maximum(np.abs(f['sad']), 1e-300)) <= 1e-15
-    d = f['btd_A'].copy()
-    l = np.zeros(nzi * ns * ns)
-    piv = np.zeros(nel, dtype=np.int32)
-    py_btddod_full_factorize(d, nzi, ns, l, piv)
-    x = np.zeros(nv)
-    py_btddod_full_solve(d, l, piv, f['btd_b'], nzi, ns, x)
-    e = np.max(np.abs(x - f['btd_x'])) / np.max(np.abs(f['btd_x']))
-    say('btd solve err %.3e' % e)
-    assert e <= 1e-11
+    for k in ('btd', 'btdr'):  # the psitc matrix, then a random system whose factorization pivots
+        d = f[k + '_A'].copy()
+        l = np.zeros(nzi * ns * ns)
+        piv = np.zeros(nel, dtype=np.int32)
+        py_btddod_full_factorize(d, nzi, ns, l, piv)
+        if k == 'btdr':
+            assert np.array_equal(piv[:nv], f['btdr_piv'][:nv])
+        x = np.zeros(nv)
+        py_btddod_full_solve(d, l, piv, f['btd_b'], nzi, ns, x)
+        e = np.max(np.abs(x - f[k + '_x'])) / np.max(np.abs(f[k + '_x']))
+        say(k, 'solve err %.3e' % e)
+        assert e <= 1e-11
 
 
 def test_batched_btd_ragged_matches_oracle(mech_53_data, oracle_lib):

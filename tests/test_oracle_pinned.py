@@ -125,13 +125,16 @@ def test_oracle_flamelet_matches_reference_fixtures(fx, mech, request, oracle_li
     expeig = o.flamelet_jacobian(*args, compute_eig=True, diffterm=20.)[1]
     assert rel_err(expeig, f['expeig'], 1e-300) <= 1e-8
     # block Thomas: same solution as the reference and as a dense LAPACK solve of the assembled matrix
-    d, l, piv = o.btddod_full_factorize(f['btd_A'], nzi, ns)
-    x = o.btddod_full_solve(d, l, piv, f['btd_b'], nzi, ns)
-    A = _dense_btd(f['btd_A'], nzi, ns)
-    x_np = np.linalg.solve(A, f['btd_b'])
-    scale = np.max(np.abs(x_np))
-    assert np.max(np.abs(x - f['btd_x'])) <= 1e-10 * scale
-    assert np.max(np.abs(x - x_np)) <= 1e-10 * scale
+    for k in ('btd', 'btdr'):  # the psitc matrix, then a random system whose factorization pivots
+        d, l, piv = o.btddod_full_factorize(f[k + '_A'], nzi, ns)
+        if k == 'btdr':
+            assert np.array_equal(piv, f['btdr_piv'])
+        x = o.btddod_full_solve(d, l, piv, f['btd_b'], nzi, ns)
+        A = _dense_btd(f[k + '_A'], nzi, ns)
+        x_np = np.linalg.solve(A, f['btd_b'])
+        scale = np.max(np.abs(x_np))
+        assert np.max(np.abs(x - f[k + '_x'])) <= 1e-12 * scale
+        assert np.max(np.abs(x - x_np)) <= 1e-12 * scale
     assert rel_err(o.btddod_scale_and_add_diagonal(f['jac_00'], -1., f['sad_diag'], 1., nzi, ns), f['sad'],
                    1e-300) <= 1e-15
 

@@ -14,9 +14,9 @@ Rate parameters follow the workload definitions: A log-uniform in [1e6, 1e14] in
 (converted with A_SI = A_cgs * 1e-3^(order-1)), b uniform in [-1.5, 1.5], Ea uniform in
 [0, 200] kJ/mol; reverse rates come from equilibrium constants. Heat capacities are smooth,
 monotone seeded cp(T) curves, least-squares fitted by NASA7 polynomials on [300, 1000] K and
-[1000, 3500] K with cp, h and s continuous at 1000 K. Formation enthalpies follow a seeded
-bond-class model (atomization enthalpies minus H-X and heavy-heavy bond energies in textbook ranges), so that
-H2/O2 mixtures are exothermic (H2O-like products are stable).
+[1000, 3500] K with cp, h and s continuous at 1000 K. Formation enthalpies of the named species are
+approximate tabulated values; the generic species follow a seeded bond-class model (atomization enthalpies
+minus H-X and heavy-heavy bond energies in textbook ranges), so H2/O2 mixtures release realistic heat.
 Every reaction conserves elements, so mass production rates sum to zero, and is written in its exothermic
 direction (recombinations as A + B (+M) -> C), so reverse rates from equilibrium constants stay bounded.
 """
@@ -35,7 +35,32 @@ _NAMED = [('H2', (2, 0, 0)), ('O2', (0, 2, 0)), ('H2O', (2, 1, 0)), ('H', (1, 0,
           ('N2H4', (4, 0, 2)), ('H2NO', (2, 1, 1)), ('HNO2', (1, 2, 1)), ('O3', (0, 3, 0)), ('NO3', (0, 3, 1)),
           ('HNO3', (1, 3, 1)), ('H2N2O', (2, 1, 2)), ('NH2OH', (3, 1, 1))]
 _ATOMIZATION = {'H': 218.0, 'O': 249.2, 'N': 472.7}  # kJ/mol, elements -> gaseous atoms
+# approximate tabulated standard enthalpies of formation at 298.15 K (kJ/mol) of the named species, so that
+# H2/air burns with a realistic heat release
+_H298_NAMED = {'H2O': -241.8, 'H': 218.0, 'O': 249.2, 'OH': 37.3, 'HO2': 12.0, 'H2O2': -136.1, 'NO': 91.3,
+               'N': 472.7, 'NH': 358.0, 'NH2': 189.0, 'NH3': -45.9, 'NO2': 33.1, 'N2O': 82.1, 'HNO': 106.8,
+               'NNH': 249.5, 'HONO': -78.6, 'N2H2': 212.9, 'N2H3': 222.0, 'N2H4': 95.4, 'H2NO': 66.0,
+               'HNO2': -44.0, 'O3': 142.7, 'NO3': 73.7, 'HNO3': -134.3, 'H2N2O': 80.0, 'NH2OH': -40.0}
 _STANDARD = {(2, 0, 0), (0, 2, 0), (0, 0, 2)}
+
+# A hydrogen-oxidation core with approximate literature rate parameters (A in cgs mol-cm-s units, b,
+# Ea in kJ/mol) that every mechanism starts from, so that H2/air flames ignite and burn; the remaining
+# reactions are random. Three-body efficiencies: H2 2.5, H2O 12.
+_CORE_SIMPLE = [(('H', 'O2'), ('O', 'OH'), 1.04e14, 0.0, 63.96),
+                (('O', 'H2'), ('H', 'OH'), 3.82e12, 0.0, 33.26),
+                (('H2', 'OH'), ('H2O', 'H'), 2.16e8, 1.51, 14.35),
+                (('OH', 'OH'), ('O', 'H2O'), 3.34e4, 2.42, -8.07),
+                (('HO2', 'H'), ('H2', 'O2'), 2.75e6, 2.09, -6.0),
+                (('HO2', 'H'), ('OH', 'OH'), 7.08e13, 0.0, 1.23),
+                (('HO2', 'OH'), ('H2O', 'O2'), 2.89e13, 0.0, -2.09),
+                (('HO2', 'HO2'), ('H2O2', 'O2'), 4.2e14, 0.0, 50.1),
+                (('H2O2', 'H'), ('H2O', 'OH'), 2.41e13, 0.0, 16.6)]
+_CORE_THREE_BODY = [(('H', 'H'), ('H2',), 1.0e18, -1.0, 0.0),
+                    (('O', 'O'), ('O2',), 1.2e17, -1.0, 0.0),
+                    (('H', 'OH'), ('H2O',), 2.2e22, -2.0, 0.0),
+                    (('O', 'H'), ('OH',), 4.71e18, -1.0, 0.0)]
+_CORE_TROE = [(('H', 'O2'), ('HO2',), (4.65e12, 0.44, 0.0), (6.37e20, -1.72, 2.2), (0.5, 1.0e-30, 1.0e30, 0.0)),
+              (('OH', 'OH'), ('H2O2',), (7.4e13, -0.37, 0.0), (2.3e18, -0.9, -7.1), (0.7346, 94.0, 1756.0, 5182.0))]
 
 
 def _species_set(n_species, rng):
@@ -127,6 +152,8 @@ def make_mechanism(n_species=53, n_reactions=325, n_three_body=None, n_troe=None
         natoms = sum(c)
         if c in _STANDARD:
             h298 = 0.0
+        elif nm in _H298_NAMED:
+            h298 = _H298_NAMED[nm]
         else:
             # bond-class model: H atoms bonded to heavy atoms (360-460 kJ/mol per bond, H-H 436 when there is no
             # heavy atom), heavy-heavy single/multiple bonds 150-350 kJ/mol (textbook ranges)
@@ -171,6 +198,33 @@ def make_mechanism(n_species=53, n_reactions=325, n_three_body=None, n_troe=None
 
     recomb_pairs = [(a, b) for a in active for b in active if a <= b and add(a, b) in by_comp]
     rxns = []
+    index = {nm: i for i, nm in enumerate(names)}
+    core_eff = _ordered({'H2': 2.5, 'H2O': 12.0})
+
+    def core_arr(A, bexp, Ea, order):
+        return A * (1.0e-3) ** (order - 1), bexp, Ea * 1.0e6
+
+    def cmap(nms):
+        return stoich_map([index[n] for n in nms])
+
+    core_ok = all(n in index for n in ('H2', 'O2', 'H2O', 'H', 'O', 'OH', 'HO2', 'H2O2'))
+    if core_ok:
+        for reac, prod, A, bexp, Ea in _CORE_SIMPLE[:n_simple]:
+            A, bexp, Ea = core_arr(A, bexp, Ea, 2)
+            rxns.append({'type': 'simple', 'reversible': True, 'reactants': cmap(reac), 'products': cmap(prod),
+                         'A': A, 'b': bexp, 'Ea': Ea})
+        for reac, prod, A, bexp, Ea in _CORE_THREE_BODY[:n_three_body]:
+            A, bexp, Ea = core_arr(A, bexp, Ea, 3)
+            rxns.append({'type': 'three-body', 'reversible': True, 'reactants': cmap(reac), 'products': cmap(prod),
+                         'A': A, 'b': bexp, 'Ea': Ea, 'efficiencies': core_eff, 'default-eff': 1.0})
+        for reac, prod, ki, k0, troe in _CORE_TROE[:n_troe]:
+            Ai, bi, Ei = core_arr(*ki, 2)
+            A0, b0, E0 = core_arr(*k0, 3)
+            rxns.append({'type': 'Troe', 'reversible': True, 'reactants': cmap(reac), 'products': cmap(prod),
+                         'fwd-A': Ai, 'fwd-b': bi, 'fwd-Ea': Ei, 'flf-A': A0, 'flf-b': b0, 'flf-Ea': E0,
+                         'efficiencies': core_eff, 'default-eff': 1.0, 'Troe-params': np.array(troe)})
+    n_core_tb = sum(1 for r in rxns if r['type'] == 'three-body')
+    n_core_troe = sum(1 for r in rxns if r['type'] == 'Troe')
     # simple reversible elementary reactions
     tries = 0
     while sum(1 for r in rxns if r['type'] == 'simple') < n_simple:
@@ -209,13 +263,13 @@ def make_mechanism(n_species=53, n_reactions=325, n_three_body=None, n_troe=None
         eff = {names[i]: float(np.round(rng.uniform(0.5, 5.0), 3)) for i in cand}
         return _ordered(eff)
 
-    for _ in range(n_three_body):
+    for _ in range(n_three_body - n_core_tb):
         a, b = recomb_pairs[rng.integers(len(recomb_pairs))]
         A, bexp, Ea = arr(3)
         rxns.append({'type': 'three-body', 'reversible': True, 'reactants': stoich_map([a, b]),
                      'products': stoich_map([by_comp[add(a, b)]]), 'A': A, 'b': bexp, 'Ea': Ea,
                      'efficiencies': efficiencies(), 'default-eff': 1.0})
-    for _ in range(n_troe):
+    for _ in range(n_troe - n_core_troe):
         a, b = recomb_pairs[rng.integers(len(recomb_pairs))]
         Ai, bi, Ei = arr(2)
         A0, b0, E0 = arr(3)

@@ -197,6 +197,8 @@ def make_mechanism(n_species=53, n_reactions=325, n_three_body=None, n_troe=None
         return _ordered(m)
 
     recomb_pairs = [(a, b) for a in active for b in active if a <= b and add(a, b) in by_comp]
+    # falloff and three-body reactions are written as recombinations A + B (+M) -> C, so only exothermic ones
+    exo_pairs = [(a, b) for a, b in recomb_pairs if h298_of[by_comp[add(a, b)]] <= h298_of[a] + h298_of[b]]
     rxns = []
     index = {nm: i for i, nm in enumerate(names)}
     core_eff = _ordered({'H2': 2.5, 'H2O': 12.0})
@@ -264,13 +266,13 @@ def make_mechanism(n_species=53, n_reactions=325, n_three_body=None, n_troe=None
         return _ordered(eff)
 
     for _ in range(n_three_body - n_core_tb):
-        a, b = recomb_pairs[rng.integers(len(recomb_pairs))]
+        a, b = exo_pairs[rng.integers(len(exo_pairs))]
         A, bexp, Ea = arr(3)
         rxns.append({'type': 'three-body', 'reversible': True, 'reactants': stoich_map([a, b]),
                      'products': stoich_map([by_comp[add(a, b)]]), 'A': A, 'b': bexp, 'Ea': Ea,
                      'efficiencies': efficiencies(), 'default-eff': 1.0})
     for _ in range(n_troe - n_core_troe):
-        a, b = recomb_pairs[rng.integers(len(recomb_pairs))]
+        a, b = exo_pairs[rng.integers(len(exo_pairs))]
         Ai, bi, Ei = arr(2)
         A0, b0, E0 = arr(3)
         troe = [float(rng.uniform(0.05, 0.95)), float(rng.uniform(50.0, 600.0)), float(rng.uniform(500.0, 6000.0)),

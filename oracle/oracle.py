@@ -236,6 +236,7 @@ class CpuGriffon:
             pre = np.zeros(nzi * n * n) if compute_eig else None
             self._f('flamelet_jacobian')(self.h, *args, int(scale_and_offset), ctypes.c_double(prefactor),
                                          int(incl_ef), int(incl_vcp), int(scaled_hl), _dp(pre), _dp(out))
+            self.last_chemistry_blocks = pre
             if compute_eig:
                 # eigenvalues of each pre-diffusion chemistry block, LAPACK dgeev via numpy
                 # (flamelet_kernels.cpp:1329-1341 calls dgeev on the same block)

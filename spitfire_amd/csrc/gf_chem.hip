@@ -5,22 +5,21 @@
 // chemistry_kernels.cpp:35-463 (production rates), rates_sensitivities_exact.cpp:33-1028 (exact
 // sensitivities), isobaric_reactor_kernels.cpp:19-98, 319-343 (RHS, primitive Jacobian, transform).
 //
-// Structure: persistent workgroups (LDS-limited, about one per CU), each owning a contiguous range of
-// states processed S at a time; all per-state intermediates live in LDS; no atomics, so results are
-// deterministic:
+// Structure: persistent workgroups (LDS-limited), each owning a contiguous range of states processed S at
+// a time. The compact gather tables of the mechanism are copied into LDS once per workgroup; all
+// per-state intermediates live in LDS; no atomics, so results are deterministic:
 //   P0  one wave per state, lane = species: mixture thermo; wave-shuffle reductions for sum(y/M), cp, dcp/dT
 //   P1  lane = (reaction, state), a reaction's S states in adjacent lanes: kf, Kc, falloff factor, mass
 //       action and all partial derivatives.
 //       dq_r/dY_s = sum_p v_{r,p} delta(s,p) + a_r (1/M_s - 1/M_last) + b_r, so each reaction stores five
-//       scalars (q, dq/drho, dq/dT, a, b), its enthalpy change H_r and one value v per participant slot.
+//       scalars (q, dq/drho, dq/dT, a, b) and one value v per participant slot.
 //   P2  one wave per state, lane = species k: gathers over the reactions with k in their net
-//       stoichiometry give w_k, dw_k/drho, dw_k/dT and the rank-1 coefficients; wave reductions give the
-//       enthalpy-weighted sums for the temperature row; the RHS is written here.
-//   P3a lane = (column, state): the temperature-row and diagonal sums of each column (the diagonal entry
-//       collects every reaction of that species, so it is summed here rather than in the entry pass).
-//   P3b lane = Jacobian entry (column-major, so stores are contiguous), looping over the S states: sparse
-//       gather over the slots of that (row species, column species) pair + rank-1 part + chem_jac_isobaric
-//       + the (rho,T,Y)->(T,Y) transform at constant pressure.
+//       stoichiometry give w_k, dw_k/drho, dw_k/dT, the rank-1 coefficients and the diagonal sensitivity
+//       entry of column k; wave reductions give the enthalpy-weighted sums; the RHS is written here.
+//   P3  one wave per Jacobian column, lane = row (plus one lane for the last species), looping over the S
+//       states: off-diagonal sparse gather + rank-1 part give the species rows (chem_jac_isobaric + the
+//       (rho,T,Y)->(T,Y) transform at constant pressure); the temperature row is the enthalpy-weighted
+//       wave sum of the column; each column of each state is one contiguous 53-double store.
 #include "gf_internal.h"
 #include "../../include/griffon_amd.h"
 
@@ -33,10 +32,11 @@ namespace gfa {
 namespace {
 
 constexpr int SP_STRIDE = NS_MAX;   // per-species LDS arrays
-constexpr int N_SPEC_ARR = 13;      // y h cpi G dBdT w ws0 ws1 A B pr SH DG
+constexpr int N_SPEC_ARR = 12;      // y h cpi G dBdT w ws0 ws1 A B pr DG
+constexpr int N_SPEC_ARR_RHS = 4;   // the RHS-only kernel keeps y h cpi G
 constexpr int N_SCAL = 16;
 
-enum { Y_ = 0, H_, CPI_, G_, DBDT_, W_, WS0_, WS1_, A_, B_, PR_, SH_, DG_ };
+enum { Y_ = 0, H_, CPI_, G_, DBDT_, W_, WS0_, WS1_, A_, B_, PR_, DG_ };
 enum { S_T = 0, S_RHO, S_MMW, S_CP, S_CPST, S_INVT, S_LOGT, S_CT, S_INVM, S_LOGPORT, S_RHS0, S_HS0, S_HS1, S_WCP,
        S_HA, S_HB };
 
@@ -127,11 +127,24 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
   const int ns = mv.ns, nr = mv.nr, nm = ns - 1;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const long stride = N_SPEC_ARR * SP_STRIDE + N_SCAL + 6L * nr + mv.nslot;
+  constexpr int NSA = JAC ? N_SPEC_ARR : N_SPEC_ARR_RHS;
+  const long stride = NSA * SP_STRIDE + N_SCAL + (JAC ? 5L * nr + mv.nslot : (long)nr);
   const long ngroups = (a.n + S - 1) / S;
   // each workgroup owns a contiguous range of state groups (neighbouring states share SoA input lines)
   const long g0 = (long)blockIdx.x * groups_per_wg;
   const long g1 = min(ngroups, g0 + groups_per_wg);
+
+  // gather tables -> LDS (the RHS-only kernel needs only the species gather)
+  const int tcopy = JAC ? mv.tbl_bytes : ((mv.exo_at + 15) & ~15);
+  for (int q = threadIdx.x; q < tcopy / 16; q += THREADS)
+    reinterpret_cast<uint4 *>(lds)[q] = reinterpret_cast<const uint4 *>(mv.tbl)[q];
+  const char *tb = reinterpret_cast<const char *>(lds);
+  const uint32_t *sgx_off = reinterpret_cast<const uint32_t *>(tb);
+  const uint64_t *sgx = reinterpret_cast<const uint64_t *>(tb + mv.sgx_at);
+  const uint16_t *exo = reinterpret_cast<const uint16_t *>(tb + mv.exo_at);
+  const uint32_t *exx = reinterpret_cast<const uint32_t *>(tb + mv.exx_at);
+  double *sbase = lds + tcopy / 8;
+  __syncthreads();
 
   for (long group = g0; group < g1; ++group) {
     const long first = group * S;
@@ -139,8 +152,8 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
     // ---------------- P0: mixture thermo (wave per state, lane = species) ----------------
     if (wave < S) {
       const long i = first + wave;
-      double *sp = lds + wave * stride;
-      double *sc = sp + N_SPEC_ARR * SP_STRIDE;
+      double *sp = sbase + wave * stride;
+      double *sc = sp + NSA * SP_STRIDE;
       if (i < a.n) {
         const double *xi = a.x + i * a.xs_i;
         const double T = xi[0];
@@ -163,7 +176,7 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
           sp[H_ * SP_STRIDE + lane] = h;
           sp[CPI_ * SP_STRIDE + lane] = cp;
           sp[G_ * SP_STRIDE + lane] = Gk;
-          sp[DBDT_ * SP_STRIDE + lane] = dB;
+          if (JAC) sp[DBDT_ * SP_STRIDE + lane] = dB;
         }
         const double cpm = wave_sum(lane < ns ? yk * cp : 0.);
         // a constant-cp species resets the dcp/dT accumulation (thermodynamics_kernels.cpp:200-203)
@@ -184,10 +197,10 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
       const int r = it / S, s = it - r * S;
       const long i = first + s;
       if (i >= a.n) continue;
-      double *sp = lds + s * stride;
-      const double *sc = sp + N_SPEC_ARR * SP_STRIDE;
-      double *rq = sp + N_SPEC_ARR * SP_STRIDE + N_SCAL;
-      double *v = rq + 6 * nr;
+      double *sp = sbase + s * stride;
+      const double *sc = sp + NSA * SP_STRIDE;
+      double *rq = sp + NSA * SP_STRIDE + N_SCAL;
+      double *v = rq + 5 * nr;
       const double *y = sp + Y_ * SP_STRIDE;
       const double T = sc[S_T], rho = sc[S_RHO], invT = sc[S_INVT], logT = sc[S_LOGT], ct = sc[S_CT],
                    invM = sc[S_INVM];
@@ -360,39 +373,38 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
         if (slot >= 0) vr[slot] += Rnet * tbs * e;
         else if (q == R.tb_last) Elast = tbs * e;
       }
-      double Hr = 0.;
-      for (int q = 0; q < R.n_net; ++q) {
-        const int k = R.net_idx[q];
-        Hr += (-R.net_st[q] * mv.mw[k]) * sp[H_ * SP_STRIDE + k];
-      }
       rq[r] = Rnet * Ctbaf;
       rq[nr + r] = dRdrho * Ctbaf + dCdrho * Rnet;
       rq[2 * nr + r] = dRdT * Ctbaf + dCdT * Rnet;
       rq[3 * nr + r] = Rnet * cdense;
       rq[4 * nr + r] = Ctbaf * U - Rnet * Elast;
-      rq[5 * nr + r] = Hr;
     }
     __syncthreads();
 
     // ---------------- P2: species gathers + reductions (wave per state, lane = species) ----------------
     if (wave < S) {
       const long i = first + wave;
-      double *sp = lds + wave * stride;
-      double *sc = sp + N_SPEC_ARR * SP_STRIDE;
+      double *sp = sbase + wave * stride;
+      double *sc = sp + NSA * SP_STRIDE;
       const double *rq = sc + N_SCAL;
+      const double *v = rq + 5 * nr;
       if (i < a.n) {
-        double w = 0., w0 = 0., w1 = 0., A = 0., B = 0.;
+        double w = 0., w0 = 0., w1 = 0., A = 0., B = 0., DG = 0.;
         if (lane < ns) {
-          const int e0 = mv.sg_off[lane], e1 = mv.sg_off[lane + 1];
-          for (int e = e0; e < e1; ++e) {
-            const int r = mv.sg_r[e];
-            const double c = mv.sg_coef[e];
+          const double Mk = mv.mw[lane];
+          const uint32_t e0 = sgx_off[lane], e1 = sgx_off[lane + 1];
+          for (uint32_t e = e0; e < e1; ++e) {
+            const uint64_t rec = sgx[e];
+            const int r = (int)(rec & 0xFFFFu);
+            const double c = -(double)(int8_t)(rec >> 32) * Mk;  // -nu * M_k (rates_sensitivities_exact.cpp:1018)
             w += c * rq[r];
             if (JAC) {
               w0 += c * rq[nr + r];
               w1 += c * rq[2 * nr + r];
               A += c * rq[3 * nr + r];
               B += c * rq[4 * nr + r];
+              const int ds = (int)((rec >> 16) & 0xFFFFu);
+              if (ds != 0xFFFF) DG += c * v[ds];
             }
           }
         }
@@ -410,108 +422,91 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
         if (JAC) {
           const double cpi = lane < ns ? sp[CPI_ * SP_STRIDE + lane] : 0.;
           const double HS0 = wave_sum(h * w0), HS1 = wave_sum(h * w1), WCP = wave_sum(w * cpi);
-          const double HA = wave_sum(h * A), HB = wave_sum(h * B);
           if (lane < ns) {
             sp[W_ * SP_STRIDE + lane] = w;
             sp[WS0_ * SP_STRIDE + lane] = w0;
             sp[WS1_ * SP_STRIDE + lane] = w1;
             sp[A_ * SP_STRIDE + lane] = A;
             sp[B_ * SP_STRIDE + lane] = B;
+            sp[DG_ * SP_STRIDE + lane] = DG;
           }
           const double iRC = 1. / (rho * cp);
           if (lane < nm) sp[PR_ * SP_STRIDE + 1 + lane] = iR * (w0 - iR * w);
           if (lane == 0) {
             sp[PR_ * SP_STRIDE] = -iRC * HS0 - iR * rhs0;
-            sc[S_RHS0] = rhs0; sc[S_HS0] = HS0; sc[S_HS1] = HS1; sc[S_WCP] = WCP; sc[S_HA] = HA; sc[S_HB] = HB;
+            sc[S_RHS0] = rhs0; sc[S_HS0] = HS0; sc[S_HS1] = HS1; sc[S_WCP] = WCP;
           }
         }
       }
     }
     if (JAC) {
       __syncthreads();
-      // ---------------- P3a: per-column sums (lane = (column, state)): temperature row and diagonal -------
-      for (int it = threadIdx.x; it < S * nm; it += THREADS) {
-        const int j = it / S, s = it - j * S;
-        if (first + s >= a.n) continue;
-        double *sp = lds + s * stride;
-        const double *rq = sp + N_SPEC_ARR * SP_STRIDE + N_SCAL;
-        const double *v = rq + 6 * nr;
-        double SH = 0., DG = 0.;
-        for (int t = mv.cg_off[j]; t < mv.cg_off[j + 1]; ++t) {
-          const double vt = v[mv.cg_slot[t]];
-          SH += vt * rq[5 * nr + mv.cg_r[t]];
-          DG += vt * mv.cg_dcoef[t];
-        }
-        sp[SH_ * SP_STRIDE + j] = SH;
-        sp[DG_ * SP_STRIDE + j] = DG;
-      }
-      __syncthreads();
-
-      // ---------------- P3b: Jacobian entries (lane = entry, column-major: contiguous stores) -------------
+      // ---------------- P3: Jacobian columns (wave = column, lane = row; lane ns = the last species) -------
       int nact = 0;
-      double iR[S], iC[S], iRC[S], roT[S], rhs0[S], rmw[S], HA[S], HB[S], T00[S];
+      double iR[S], iC[S], iRC[S], roT[S], rhs0[S], rmw[S], T00[S];
       for (int s = 0; s < S; ++s) {
         if (first + s < a.n) nact = s + 1;
-        const double *sc = lds + s * stride + N_SPEC_ARR * SP_STRIDE;
+        const double *sc = sbase + s * stride + NSA * SP_STRIDE;
         iR[s] = 1. / sc[S_RHO];
         iC[s] = 1. / sc[S_CP];
         iRC[s] = 1. / (sc[S_RHO] * sc[S_CP]);
         roT[s] = sc[S_RHO] / sc[S_T];
         rhs0[s] = sc[S_RHS0];
         rmw[s] = -sc[S_RHO] * sc[S_MMW];
-        HA[s] = sc[S_HA];
-        HB[s] = sc[S_HB];
         T00[s] = -iRC[s] * (sc[S_HS1] + sc[S_WCP]) - rhs0[s] * sc[S_CPST] * iC[s];
       }
       if (a.jac) {
-        const int nent = ns * ns;
-        for (int e = threadIdx.x; e < nent; e += THREADS) {
-          const int c = e / ns, rr = e - c * ns;
+        constexpr int NW = THREADS / 64;
+        for (int c = wave; c < ns; c += NW) {
           double val[S];
           if (c == 0) {
             for (int s = 0; s < S; ++s) {
-              const double *sp = lds + s * stride;
+              const double *sp = sbase + s * stride;
               const double *pr = sp + PR_ * SP_STRIDE;
-              val[s] = (rr == 0) ? T00[s] - roT[s] * pr[0]
-                                 : sp[WS1_ * SP_STRIDE + rr - 1] * iR[s] - roT[s] * pr[rr];
+              val[s] = (lane == 0) ? T00[s] - roT[s] * pr[0]
+                                   : (lane < ns ? sp[WS1_ * SP_STRIDE + lane - 1] * iR[s] - roT[s] * pr[lane] : 0.);
             }
           } else {
             const int j = c - 1;
             const double dj = mv.dinvm[j];
-            if (rr == 0) {
-              for (int s = 0; s < S; ++s) {
-                const double *sp = lds + s * stride;
-                val[s] = -iRC[s] * (sp[SH_ * SP_STRIDE + j] + dj * HA[s] + HB[s]) -
-                         rhs0[s] * (sp[CPI_ * SP_STRIDE + j] - sp[CPI_ * SP_STRIDE + nm]) * iC[s] +
-                         rmw[s] * dj * sp[PR_ * SP_STRIDE];
-              }
-            } else {
-              const int ii = rr - 1;
+            const int ii = lane - 1;  // species row; lane ns is the last species (temperature row only)
+            const bool row = lane >= 1 && lane <= ns;
+            double full[S];
+            for (int s = 0; s < S; ++s) full[s] = 0.;
+            if (row) {
+              for (int s = 0; s < S; ++s) full[s] = (ii == j) ? sbase[s * stride + DG_ * SP_STRIDE + j] : 0.;
+              const double Mi = mv.mw[ii];
               const int ee = j * ns + ii;
-              double acc[S];
-              for (int s = 0; s < S; ++s) acc[s] = (ii == j) ? lds[s * stride + DG_ * SP_STRIDE + j] : 0.;
-              const int t0 = mv.ex_off[ee], t1 = mv.ex_off[ee + 1];
+              const int t0 = exo[ee], t1 = exo[ee + 1];
               for (int t = t0; t < t1; ++t) {
-                const int slot = mv.ex_slot[t];
-                const double coef = mv.ex_coef[t];
-                for (int s = 0; s < S; ++s)
-                  acc[s] += coef * lds[s * stride + N_SPEC_ARR * SP_STRIDE + N_SCAL + 6 * nr + slot];
+                const uint32_t rec = exx[t];
+                const int slot = (int)(rec & 0xFFFFu);
+                const double coef = -(double)(int8_t)(rec >> 16) * Mi;
+                for (int s = 0; s < S; ++s) full[s] += coef * sbase[s * stride + NSA * SP_STRIDE + N_SCAL + 5 * nr + slot];
               }
               for (int s = 0; s < S; ++s) {
-                const double *sp = lds + s * stride;
-                val[s] = iR[s] * (acc[s] + dj * sp[A_ * SP_STRIDE + ii] + sp[B_ * SP_STRIDE + ii]) +
-                         rmw[s] * dj * sp[PR_ * SP_STRIDE + rr];
+                const double *sp = sbase + s * stride;
+                full[s] += dj * sp[A_ * SP_STRIDE + ii] + sp[B_ * SP_STRIDE + ii];
               }
             }
+            for (int s = 0; s < S; ++s) {
+              const double *sp = sbase + s * stride;
+              // temperature row: -(1/(rho cp)) sum_k h_k dw_k/dY_j over every species (isobaric_reactor_kernels.cpp:58-98)
+              const double hs = wave_sum(row ? sp[H_ * SP_STRIDE + ii] * full[s] : 0.);
+              val[s] = (lane == 0) ? -iRC[s] * hs - rhs0[s] * (sp[CPI_ * SP_STRIDE + j] - sp[CPI_ * SP_STRIDE + nm]) * iC[s] +
+                                         rmw[s] * dj * sp[PR_ * SP_STRIDE]
+                                   : (lane < ns ? iR[s] * full[s] + rmw[s] * dj * sp[PR_ * SP_STRIDE + lane] : 0.);
+            }
           }
-          for (int s = 0; s < nact; ++s) a.jac[(first + s) * a.js_i + e] = val[s];
+          if (lane < ns)
+            for (int s = 0; s < nact; ++s) a.jac[(first + s) * a.js_i + (long)c * ns + lane] = val[s];
         }
       }
       if (a.sens) {
         // raw option-0 sensitivities, (ns+1)^2 column-major with stride ns+1 (rates_sensitivities_exact.cpp:1011-1026)
         for (int s = 0; s < nact; ++s) {
-          const double *sp = lds + s * stride;
-          const double *v = sp + N_SPEC_ARR * SP_STRIDE + N_SCAL + 6 * nr;
+          const double *sp = sbase + s * stride;
+          const double *v = sp + NSA * SP_STRIDE + N_SCAL + 5 * nr;
           double *so = a.sens + (first + s) * (long)(ns + 1) * (ns + 1);
           for (int e = threadIdx.x; e < (ns + 1) * (ns + 1); e += THREADS) {
             const int c = e / (ns + 1), k = e - c * (ns + 1);
@@ -537,18 +532,25 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
 }
 
 template <int S, int THREADS, bool JAC>
+size_t lds_bytes(const gf_mech *m) {
+  const size_t t = JAC ? (size_t)m->view.tbl_bytes : (((size_t)m->view.exo_at + 15) & ~size_t(15));
+  return t + S * (JAC ? m->lds_per_state : m->lds_per_state_rhs);
+}
+
+template <int S, int THREADS, bool JAC>
 int launch_cfg(const gf_mech *m, const ChemArgs &a, hipStream_t st) {
-  const size_t lds = m->lds_per_state * S;
+  const size_t lds = lds_bytes<S, THREADS, JAC>(m);
   auto kern = chem_kernel<S, THREADS, JAC>;
+  if (lds > m->lds_max) return fail(GF_ERR_UNSUPPORTED, "mechanism needs more LDS per workgroup than the device allows");
   if (lds > 65536) {
     if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
       return fail(GF_ERR_UNSUPPORTED, "mechanism needs more LDS per workgroup than the device allows");
   }
   const long groups = (a.n + S - 1) / S;
-  // persistent launch: as many workgroups as fit the device at once (LDS-limited), each owning a
-  // contiguous range of groups
-  const long per_cu = std::max<long>(1, (long)std::max<size_t>(m->lds_max, 65536) / (long)std::max<size_t>(lds, 1));
-  const long width = std::max<long>(1, (long)std::max(m->n_cu, 1) * std::min<long>(per_cu, 8));
+  // persistent launch: as many workgroups as fit the device at once (LDS-limited, at most 2048 threads
+  // per CU), each owning a contiguous range of groups
+  const long per_cu = std::max<long>(1, std::min<long>((long)(std::max<size_t>(m->lds_max, 65536) / lds), 2048 / THREADS));
+  const long width = std::max<long>(1, (long)std::max(m->n_cu, 1) * per_cu);
   const long nwg = std::min(groups, width);
   const long gpw = (groups + nwg - 1) / nwg;
   const long grid = (groups + gpw - 1) / gpw;
@@ -560,7 +562,8 @@ int launch_cfg(const gf_mech *m, const ChemArgs &a, hipStream_t st) {
 }  // namespace
 
 // Launch configuration: states per workgroup S and threads per workgroup. GF_CHEM_CFG=<S>x<threads>
-// overrides the default (used by the launch-shape sweep).
+// overrides the default (used by the launch-shape sweep); configurations whose LDS does not fit the
+// device fall back to fewer states per workgroup.
 int launch_chem(const gf_mech *m, const ChemIO &a, hipStream_t st) {
   if (a.n <= 0) return GF_OK;
   static int cfg = -1;
@@ -574,19 +577,26 @@ int launch_chem(const gf_mech *m, const ChemIO &a, hipStream_t st) {
       else if (!strcmp(e, "2x512")) cfg = 4;
     }
   }
-  int c = cfg;
-  // fewer states per workgroup when S states do not fit the workgroup LDS limit of this device
-  const size_t lim = m->lds_max;
-  if ((c == 2 || c == 3) && m->lds_per_state * 4 > lim) c = 1;
-  if ((c == 1 || c == 4) && m->lds_per_state * 2 > lim) c = 0;
-  if (m->lds_per_state > lim) return fail(GF_ERR_UNSUPPORTED, "mechanism too large for one workgroup's LDS");
   const bool J = a.jac || a.sens;
+  int c = cfg;
+  if (J) {
+    const size_t lim = m->lds_max;
+    if ((c == 2 || c == 3) && lds_bytes<4, 512, true>(m) > lim) c = 1;
+    if ((c == 1 || c == 4) && lds_bytes<2, 256, true>(m) > lim) c = 0;
+    switch (c) {
+      case 0: return launch_cfg<1, 128, true>(m, a, st);
+      case 1: return launch_cfg<2, 256, true>(m, a, st);
+      case 3: return launch_cfg<4, 256, true>(m, a, st);
+      case 4: return launch_cfg<2, 512, true>(m, a, st);
+      default: return launch_cfg<4, 512, true>(m, a, st);
+    }
+  }
   switch (c) {
-    case 0: return J ? launch_cfg<1, 128, true>(m, a, st) : launch_cfg<1, 128, false>(m, a, st);
-    case 1: return J ? launch_cfg<2, 256, true>(m, a, st) : launch_cfg<2, 256, false>(m, a, st);
-    case 3: return J ? launch_cfg<4, 256, true>(m, a, st) : launch_cfg<4, 256, false>(m, a, st);
-    case 4: return J ? launch_cfg<2, 512, true>(m, a, st) : launch_cfg<2, 512, false>(m, a, st);
-    default: return J ? launch_cfg<4, 512, true>(m, a, st) : launch_cfg<4, 512, false>(m, a, st);
+    case 0: return launch_cfg<1, 128, false>(m, a, st);
+    case 1: return launch_cfg<2, 256, false>(m, a, st);
+    case 4: return launch_cfg<2, 512, false>(m, a, st);
+    case 2: return launch_cfg<4, 512, false>(m, a, st);
+    default: return launch_cfg<4, 256, false>(m, a, st);
   }
 }
 

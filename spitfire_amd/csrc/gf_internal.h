@@ -7,7 +7,8 @@
 
 namespace gfa {
 
-constexpr int NS_MAX = 64;     // species per mechanism handled by the wave-per-state kernels (one lane per species)
+constexpr int NS_MAX = 64;     // species-array stride in the wave-per-state kernels (one lane per species)
+constexpr int NS_LIMIT = 63;   // species per mechanism (the Jacobian pass uses one extra lane for the last species)
 constexpr int NRC_MAX = 4;     // reactants per reaction
 constexpr int NPD_MAX = 4;     // products per reaction
 constexpr int NNET_MAX = 8;    // net species per reaction (griffon: 2..8)
@@ -59,24 +60,20 @@ struct MechView {
   const int *sp_idx;        // special-order lists
   const double *sp_ord;
   const double *sp_invmw;
-  // species <- reactions gather (net stoichiometry): w_k = sum coef * q_r
-  const int *sg_off;        // [ns+1]
-  const int *sg_r;
-  const double *sg_coef;    // -net_stoich * net_mw  (rates_sensitivities_exact.cpp:1018)
-  // sensitivity entry (species row i < ns, species column j < ns-1) <- slots
-  const int *eg_off;        // [(ns-1)*ns+1], entry index e = j*ns + i
+  // sensitivity entry (species row i < ns, species column j < ns-1) <- slots: entry e = j*ns + i
+  // (drop-in raw sensitivities only)
+  const int *eg_off;        // [(ns-1)*ns+1]
   const int *eg_slot;
   const double *eg_coef;
-  // the same without the diagonal terms (i == j), which are summed per column with cg_dcoef
-  const int *ex_off;
-  const int *ex_slot;
-  const double *ex_coef;
-  // column j <- slots with species j: the temperature row (weight: the reaction's enthalpy change H_r)
-  // and the diagonal entry (weight cg_dcoef: -net_stoich_j * mw_j, or 0 if j is not a net species)
-  const int *cg_off;        // [ns]
-  const int *cg_slot;
-  const int *cg_r;
-  const double *cg_dcoef;
+  // Compact gather tables, copied into LDS by the chemistry kernel (byte image at tbl, tbl_bytes long):
+  //   u32 sgx_off[ns+1]; u64 sgx[n_sgx]   species k <- reactions with k in their net stoichiometry:
+  //       bits 0-15 reaction, 16-31 slot of species k in that reaction (0xFFFF: none), 32-39 net
+  //       stoichiometric coefficient nu (int8); the gather coefficient is -nu * M_k
+  //   u16 exx_off[(ns-1)*ns+1]; u32 exx[n_exx]   off-diagonal sensitivity entry (i, j), e = j*ns + i, <-
+  //       slots of species j: bits 0-15 slot, 16-23 nu of species i (int8); coefficient -nu * M_i
+  const char *tbl;
+  int tbl_bytes;
+  int sgx_at, exo_at, exx_at;  // byte offsets of sgx, exx_off and exx inside the table image
 };
 
 struct Species {
@@ -111,7 +108,8 @@ struct gf_mech {
   int device = -1;
   void *dblob = nullptr;          // device copy
   gfa::MechView view;             // device pointers
-  size_t lds_per_state = 0;       // bytes of LDS the chemistry kernels need per state
+  size_t lds_per_state = 0;       // bytes of LDS the Jacobian chemistry kernel needs per state
+  size_t lds_per_state_rhs = 0;   // the same for the RHS-only kernel
   int n_cu = 0;                   // compute units of the device (persistent launch width)
   size_t lds_max = 65536;         // largest LDS allocation one workgroup may use on this device
   // staging buffers for the host-pointer drop-ins

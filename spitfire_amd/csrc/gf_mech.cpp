@@ -39,7 +39,7 @@ struct Arena {
 int finalize(gf_mech *m) {
   const int ns = (int)m->species.size();
   if (ns < 2) return fail(GF_ERR_STATE, "mechanism needs at least two species");
-  if (ns > NS_MAX) return fail(GF_ERR_UNSUPPORTED, "more than 64 species is not supported");
+  if (ns > NS_LIMIT) return fail(GF_ERR_UNSUPPORTED, "more than 63 species is not supported");
   for (int i = 0; i < ns; ++i)
     if (m->species[i].cpkind == 0) return fail(GF_ERR_STATE, "species " + std::to_string(i) + " has no heat capacity data");
   const int nr = (int)m->rxns.size();
@@ -148,61 +148,68 @@ int finalize(gf_mech *m) {
   }
   const int nslot = (int)slot_sp.size();
 
-  // gathers
-  std::vector<int> sg_off(ns + 1, 0), sg_r;
-  std::vector<double> sg_coef;
+  const int nm = ns - 1;
+  // species <- reactions (net stoichiometry) with the slot of that species in the reaction, if any
+  std::vector<uint32_t> sgx_off(ns + 1, 0);
+  std::vector<uint64_t> sgx;
   {
-    std::vector<std::vector<std::pair<int, double>>> L(ns);
-    for (int r = 0; r < nr; ++r)
-      for (int q = 0; q < drx[r].n_net; ++q) {
-        const int k = drx[r].net_idx[q];
-        L[k].push_back({r, -drx[r].net_st[q] * mw[k]});
+    std::vector<std::vector<uint64_t>> L(ns);
+    for (int r = 0; r < nr; ++r) {
+      const DevRxn &d = drx[r];
+      for (int q = 0; q < d.n_net; ++q) {
+        const int k = d.net_idx[q];
+        uint64_t dslot = 0xFFFF;
+        for (int p = 0; p < d.n_slot; ++p)
+          if (slot_sp[d.slot_off + p] == k) dslot = (uint64_t)(d.slot_off + p);
+        L[k].push_back((uint64_t)r | (dslot << 16) | ((uint64_t)(uint8_t)d.net_st[q] << 32));
       }
+    }
     for (int k = 0; k < ns; ++k) {
-      sg_off[k + 1] = sg_off[k] + (int)L[k].size();
-      for (auto &e : L[k]) { sg_r.push_back(e.first); sg_coef.push_back(e.second); }
+      sgx_off[k + 1] = sgx_off[k] + (uint32_t)L[k].size();
+      sgx.insert(sgx.end(), L[k].begin(), L[k].end());
     }
   }
-  const int nm = ns - 1;
-  // eg: all terms of sensitivity entry (i, j); ex: the same without the diagonal terms i == j, which the
-  // kernels sum per column together with the temperature row (cg lists, cg_dcoef).
-  std::vector<int> eg_off(nm * ns + 1, 0), eg_slot, ex_off(nm * ns + 1, 0), ex_slot, cg_off(nm + 1, 0), cg_slot, cg_r;
-  std::vector<double> eg_coef, ex_coef, cg_dcoef;
+  // eg: all terms of raw sensitivity entry (i, j); exx: the off-diagonal ones in compact form (the
+  // diagonal terms are summed in the species gather above)
+  std::vector<int> eg_off(nm * ns + 1, 0), eg_slot;
+  std::vector<double> eg_coef;
+  std::vector<uint16_t> exx_off(nm * ns + 1, 0);
+  std::vector<uint32_t> exx;
   {
-    std::vector<std::vector<std::pair<int, double>>> E(nm * ns);
-    std::vector<std::vector<std::pair<int, int>>> C(nm);
-    std::vector<std::vector<double>> CD(nm);
+    std::vector<std::vector<std::pair<int, int>>> E(nm * ns);  // (slot, nu of the row species)
     for (int r = 0; r < nr; ++r) {
       const DevRxn &d = drx[r];
       for (int p = 0; p < d.n_slot; ++p) {
         const int j = slot_sp[d.slot_off + p];
-        double dco = 0.;
-        for (int q = 0; q < d.n_net; ++q) {
-          const int i = d.net_idx[q];
-          E[j * ns + i].push_back({d.slot_off + p, -d.net_st[q] * mw[i]});
-          if (i == j) dco = -d.net_st[q] * mw[i];
-        }
-        C[j].push_back({d.slot_off + p, r});
-        CD[j].push_back(dco);
+        for (int q = 0; q < d.n_net; ++q) E[j * ns + d.net_idx[q]].push_back({d.slot_off + p, d.net_st[q]});
       }
     }
     for (int e = 0; e < nm * ns; ++e) {
+      const int i = e % ns;
       eg_off[e + 1] = eg_off[e] + (int)E[e].size();
-      for (auto &t : E[e]) { eg_slot.push_back(t.first); eg_coef.push_back(t.second); }
-      const bool diag = (e / ns) == (e % ns);
-      ex_off[e + 1] = ex_off[e] + (diag ? 0 : (int)E[e].size());
+      for (auto &t : E[e]) { eg_slot.push_back(t.first); eg_coef.push_back(-t.second * mw[i]); }
+      const bool diag = (e / ns) == i;
       if (!diag)
-        for (auto &t : E[e]) { ex_slot.push_back(t.first); ex_coef.push_back(t.second); }
-    }
-    for (int j = 0; j < nm; ++j) {
-      cg_off[j + 1] = cg_off[j] + (int)C[j].size();
-      for (size_t q = 0; q < C[j].size(); ++q) {
-        cg_slot.push_back(C[j][q].first);
-        cg_r.push_back(C[j][q].second);
-        cg_dcoef.push_back(CD[j][q]);
-      }
+        for (auto &t : E[e]) exx.push_back((uint32_t)t.first | ((uint32_t)(uint8_t)(int8_t)t.second << 16));
+      if (exx.size() > 0xFFFF) return fail(GF_ERR_UNSUPPORTED, "too many Jacobian sensitivity terms");
+      exx_off[e + 1] = (uint16_t)exx.size();
     }
   }
+  if (nslot > 0xFFFE || nr > 0xFFFF || sgx.size() > 0xFFFFFFu)
+    return fail(GF_ERR_UNSUPPORTED, "mechanism too large for the compact gather tables");
+  // byte image of the LDS tables
+  std::vector<char> tbl;
+  auto tput = [&](const void *src, size_t bytes) {
+    const size_t at = (tbl.size() + 7) & ~size_t(7);
+    tbl.resize(at + bytes);
+    if (bytes) memcpy(tbl.data() + at, src, bytes);
+    return (int)at;
+  };
+  tput(sgx_off.data(), sgx_off.size() * sizeof(uint32_t));
+  const int sgx_at = tput(sgx.data(), sgx.size() * sizeof(uint64_t));
+  const int exo_at = tput(exx_off.data(), exx_off.size() * sizeof(uint16_t));
+  const int exx_at = tput(exx.data(), exx.size() * sizeof(uint32_t));
+  tbl.resize((tbl.size() + 15) & ~size_t(15));
 
   Arena a;
   MechView &v = m->view_host;
@@ -225,22 +232,16 @@ int finalize(gf_mech *m) {
   v.sp_idx = (const int *)P(a.put(sp_idx));
   v.sp_ord = (const double *)P(a.put(sp_ord));
   v.sp_invmw = (const double *)P(a.put(sp_invmw));
-  v.sg_off = (const int *)P(a.put(sg_off));
-  v.sg_r = (const int *)P(a.put(sg_r));
-  v.sg_coef = (const double *)P(a.put(sg_coef));
   v.eg_off = (const int *)P(a.put(eg_off));
   v.eg_slot = (const int *)P(a.put(eg_slot));
   v.eg_coef = (const double *)P(a.put(eg_coef));
-  v.ex_off = (const int *)P(a.put(ex_off));
-  v.ex_slot = (const int *)P(a.put(ex_slot));
-  v.ex_coef = (const double *)P(a.put(ex_coef));
-  v.cg_off = (const int *)P(a.put(cg_off));
-  v.cg_slot = (const int *)P(a.put(cg_slot));
-  v.cg_r = (const int *)P(a.put(cg_r));
-  v.cg_dcoef = (const double *)P(a.put(cg_dcoef));
+  v.tbl = (const char *)P(a.put(tbl));
+  v.tbl_bytes = (int)tbl.size();
+  v.sgx_at = sgx_at; v.exo_at = exo_at; v.exx_at = exx_at;
   m->blob.swap(a.buf);
   m->drx = drx;
-  m->lds_per_state = sizeof(double) * (13 * NS_MAX + 16 + 6 * (size_t)nr + (size_t)nslot);
+  m->lds_per_state = sizeof(double) * (12 * NS_MAX + 16 + 5 * (size_t)nr + (size_t)nslot);
+  m->lds_per_state_rhs = sizeof(double) * (4 * NS_MAX + 16 + (size_t)nr);
   m->finalized = true;
   return GF_OK;
 }
@@ -280,11 +281,8 @@ int ensure_device(gf_mech *m) {
   v.Tmin = rebase(v.Tmin, b); v.Tmax = rebase(v.Tmax, b); v.cpkind = rebase(v.cpkind, b); v.rx = rebase(v.rx, b);
   v.tb_idx = rebase(v.tb_idx, b); v.tb_eff = rebase(v.tb_eff, b); v.tb_slot = rebase(v.tb_slot, b);
   v.sp_idx = rebase(v.sp_idx, b); v.sp_ord = rebase(v.sp_ord, b); v.sp_invmw = rebase(v.sp_invmw, b);
-  v.sg_off = rebase(v.sg_off, b); v.sg_r = rebase(v.sg_r, b); v.sg_coef = rebase(v.sg_coef, b);
   v.eg_off = rebase(v.eg_off, b); v.eg_slot = rebase(v.eg_slot, b); v.eg_coef = rebase(v.eg_coef, b);
-  v.ex_off = rebase(v.ex_off, b); v.ex_slot = rebase(v.ex_slot, b); v.ex_coef = rebase(v.ex_coef, b);
-  v.cg_off = rebase(v.cg_off, b); v.cg_slot = rebase(v.cg_slot, b); v.cg_r = rebase(v.cg_r, b);
-  v.cg_dcoef = rebase(v.cg_dcoef, b);
+  v.tbl = rebase(v.tbl, b);
   m->view = v;
   if (!m->stream) {
     if (hipStreamCreate(&m->stream) != hipSuccess) return fail(GF_ERR_HIP, "hipStreamCreate failed");

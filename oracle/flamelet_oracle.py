@@ -51,14 +51,11 @@ class OracleFlamelet(Flamelet):
         values[:] = self._cpu.btddod_scale_and_add_diagonal(values, a, diag, b, self._nz_interior,
                                                            self._n_equations)
 
-
-class OracleFlameletEig(OracleFlamelet):
-    """The reference's pseudo-transient step limiter: expeig from LAPACK dgeev of every chemistry block
-    (flamelet_kernels.cpp:1329-1341, through numpy.linalg.eigvals)."""
-
-    def eigenvalue_bound(self, state_interior, diffterm):
+    def _jac_and_expeig(self, state_interior, diffterm):
+        # the reference's pseudo-transient step limiter: expeig from LAPACK dgeev of every chemistry block
+        # (flamelet_kernels.cpp:1329-1341, through numpy.linalg.eigvals)
         return self._cpu.flamelet_jacobian(np.ascontiguousarray(state_interior), self._pressure, self._state_oxy,
                                            self._state_fuel, self._nz_interior, self._stencils(), self._x,
                                            compute_eig=True, diffterm=diffterm,
                                            incl_ef=self._include_enthalpy_flux,
-                                           incl_vcp=self._include_variable_cp)[1]
+                                           incl_vcp=self._include_variable_cp)

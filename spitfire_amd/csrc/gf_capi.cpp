@@ -266,9 +266,7 @@ int gf_flamelet_jacobian(gf_mech *m, const double *state, double pressure, const
                          const double *nco, const double *chi, int compute_eigenvalues, double diffterm,
                          int scale_and_offset, double prefactor, int rates_opt, int transform_opt, int incl_ef,
                          int incl_vcp, int scaled_hl, double *out_expeig, double *out_jac) {
-  (void)diffterm; (void)out_expeig;
   if (nzi < 2) return fail(GF_ERR_ARG, "a flamelet needs at least two interior points");
-  if (compute_eigenvalues) return fail(GF_ERR_UNSUPPORTED, "per-block eigenvalues are not computed on the GPU");
   GF_TRY(check_options(0, 0, rates_opt, transform_opt));
   const int ns = gf_mech_n_species(m);
   const size_t bsz = (size_t)ns * ns, nbl = (size_t)nzi * bsz, noff = (size_t)(nzi - 1) * ns;
@@ -277,9 +275,9 @@ int gf_flamelet_jacobian(gf_mech *m, const double *state, double pressure, const
   FlameletIO f;
   double *dpts, *dp;
   GF_TRY(flamelet_stage(m, s, state, pressure, oxy, fuel, adiabatic, Tc, hc, Tr, hr, nzi, cmaj, csub, csup, mco, nco,
-                        chi, rup(nel * 8) + rup((size_t)nzi * ns * 8), f, dpts, dp));
+                        chi, rup(nel * 8) + 2 * rup((size_t)nzi * ns * 8), f, dpts, dp));
   f.incl_ef = incl_ef; f.incl_vcp = incl_vcp; f.scaled_hl = scaled_hl;
-  double *dj = s.take<double>(nel), *dr = s.take<double>((size_t)nzi * ns);
+  double *dj = s.take<double>(nel), *dr = s.take<double>((size_t)nzi * ns), *dex = s.take<double>((size_t)nzi * ns);
   hipStream_t st = m->stream;
   // off-diagonal entries accumulate onto the caller's array as in the reference (flamelet_kernels.cpp:1385-1394)
   GF_TRY(h2d(dj + nbl, out_jac + nbl, 2 * noff * 8, st));
@@ -287,7 +285,16 @@ int gf_flamelet_jacobian(gf_mech *m, const double *state, double pressure, const
   io.n = nzi; io.x = dpts + ns; io.xs_i = ns; io.xs_v = 1; io.p = dp; io.ps = 0;
   io.rhs = dr; io.rs_i = ns; io.rs_v = 1; io.jac = dj; io.js_i = (long)bsz;
   GF_TRY(launch_chem(m, io, st));
-  GF_TRY(launch_flamelet_jac_epilogue(m, f, scale_and_offset, prefactor, dj, st));
+  if (compute_eigenvalues) {
+    // eigenvalues of each chemistry block after the heat-loss terms, before the diffusion terms
+    // (flamelet_kernels.cpp:1290-1341)
+    if (!adiabatic) GF_TRY(launch_flamelet_jac_epilogue(m, f, 0, 0., dj, 1, st));
+    GF_TRY(launch_expeig(m, dj, (long)bsz, nzi, ns, diffterm, dex, nullptr, st));
+    GF_TRY(launch_flamelet_jac_epilogue(m, f, scale_and_offset, prefactor, dj, 2, st));
+    GF_TRY(d2h(out_expeig, dex, (size_t)nzi * ns * 8, st));
+  } else {
+    GF_TRY(launch_flamelet_jac_epilogue(m, f, scale_and_offset, prefactor, dj, 0, st));
+  }
   GF_TRY(d2h(out_jac, dj, nel * 8, st));
   return sync(st);
 }

@@ -1,0 +1,238 @@
+// Largest real part of the eigenvalues of a small dense nonsymmetric matrix, for the pseudo-time limiter
+// of the flamelet solvers.
+//
+// The reference computes every eigenvalue of each ns x ns chemistry block with LAPACK dgeev (no
+// eigenvectors; blas_lapack_kernels.h:142-180, called at flamelet_kernels.cpp:1329-1341), i.e. balancing
+// (dgebal), Householder reduction to upper Hessenberg form (dgehrd) and the Francis double-shift QR
+// iteration (dhseqr, which for n < 75 is the small-bulge dlahqr). This file restates the same three
+// published steps for one 64-lane wave per matrix:
+//   - diagonal balancing by powers of two (Parlett & Reinsch, Numer. Math. 13, 1969), which is exact;
+//   - Householder reduction to upper Hessenberg form;
+//   - the Francis implicit double-shift QR iteration on the Hessenberg matrix (Wilkinson & Reinsch,
+//     Handbook for Automatic Computation II, 1971, procedure hqr), with exceptional shifts every 10
+//     iterations on one window, deflating 1x1 and 2x2 blocks and tracking only the largest real part.
+// Every scalar decision is taken from values that all lanes read from the same LDS words or from
+// reductions that end in a lane-uniform value, so control flow stays uniform across the wave.
+//
+// The algorithm is written once against a small "lanes" policy: the device policy runs a lane loop as
+// one lane per index plus a barrier, the host policy as a plain loop, so the same code is unit-tested on
+// the CPU.
+#pragma once
+#include <cmath>
+
+#if defined(__HIPCC__)
+#define GF_HD __host__ __device__
+#else
+#define GF_HD
+#endif
+
+namespace gfa {
+
+// Matrix storage: row-major a[i * ld + j], ld odd (fewer LDS bank conflicts for column walks).
+template <class P>
+GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld, int *failed) {
+  using std::fabs;
+  using std::sqrt;
+  *failed = 0;
+  if (n <= 0) return 0.;
+  if (n == 1) return a[0];
+
+  // 1) balancing: scale row i by 1/f and column i by f (f a power of 2) until row and column norms agree
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    bool changed = false;
+    for (int i = 0; i < n; ++i) {
+      double c = par.sum(0, n, [&](int t) { return t == i ? 0. : fabs(a[t * ld + i]); });
+      double r = par.sum(0, n, [&](int t) { return t == i ? 0. : fabs(a[i * ld + t]); });
+      if (c == 0. || r == 0.) continue;
+      double g = r * 0.5, f = 1.;
+      const double s = c + r;
+      int guard = 0;
+      while (c < g && guard++ < 2100) { f *= 2.; c *= 4.; }
+      g = r * 2.;
+      guard = 0;
+      while (c > g && guard++ < 2100) { f *= 0.5; c *= 0.25; }
+      if ((c + r) / f < 0.95 * s) {
+        changed = true;
+        const double gi = 1. / f;
+        par.each(0, n, [&](int t) { a[i * ld + t] *= gi; });
+        par.each(0, n, [&](int t) { a[t * ld + i] *= f; });
+      }
+    }
+    if (!changed) break;
+  }
+
+  // 2) Householder reduction to upper Hessenberg form: column k is reduced below its subdiagonal by
+  //    H = I - beta v v^T acting on rows/columns k+1..n-1
+  for (int k = 0; k < n - 2; ++k) {
+    const double x0 = a[(k + 1) * ld + k];
+    const double tail = par.sum(k + 2, n, [&](int t) { const double x = a[t * ld + k]; return x * x; });
+    if (tail == 0.) continue;
+    const double nrm = sqrt(x0 * x0 + tail);
+    const double alpha = x0 >= 0. ? -nrm : nrm;
+    const double v0 = x0 - alpha;
+    const double beta = -1. / (alpha * v0);
+    par.each(0, n, [&](int t) { vbuf[t] = t == k + 1 ? v0 : (t > k + 1 ? a[t * ld + k] : 0.); });
+    // left: rows k+1..n-1 of columns k+1..n-1 (column k becomes alpha e1 exactly)
+    par.each(k + 1, n, [&](int j) {
+      double w = 0.;
+      for (int i = k + 1; i < n; ++i) w += vbuf[i] * a[i * ld + j];
+      w *= beta;
+      for (int i = k + 1; i < n; ++i) a[i * ld + j] -= vbuf[i] * w;
+    });
+    par.each(0, n, [&](int t) {
+      if (t == k + 1) a[t * ld + k] = alpha;
+      else if (t > k + 1) a[t * ld + k] = 0.;
+    });
+    // right: all rows, columns k+1..n-1
+    par.each(0, n, [&](int i) {
+      double u = 0.;
+      for (int j = k + 1; j < n; ++j) u += a[i * ld + j] * vbuf[j];
+      u *= beta;
+      for (int j = k + 1; j < n; ++j) a[i * ld + j] -= u * vbuf[j];
+    });
+  }
+
+  // 3) Francis double-shift QR on the Hessenberg matrix (eigenvalues only, active window [l, nn])
+  const double anorm = par.sum(0, n, [&](int i) {
+    double s = 0.;
+    for (int j = (i > 0 ? i - 1 : 0); j < n; ++j) s += fabs(a[i * ld + j]);
+    return s;
+  });
+  double emax = -INFINITY;
+  int nn = n - 1;
+  double t = 0.;  // accumulated exceptional shift
+  int total = 0;
+  const int total_max = 30 * (n > 10 ? n : 10);
+  while (nn >= 0) {
+    int its = 0, l;
+    for (;;) {
+      // find a negligible subdiagonal element
+      for (l = nn; l >= 1; --l) {
+        double s = fabs(a[(l - 1) * ld + l - 1]) + fabs(a[l * ld + l]);
+        if (s == 0.) s = anorm;
+        if (fabs(a[l * ld + l - 1]) + s == s) break;
+      }
+      const double x = a[nn * ld + nn];
+      if (l == nn) {  // one real root
+        emax = fmax(emax, x + t);
+        nn -= 1;
+        break;
+      }
+      const double y = a[(nn - 1) * ld + nn - 1];
+      const double w = a[nn * ld + nn - 1] * a[(nn - 1) * ld + nn];
+      if (l == nn - 1) {  // a pair of roots from the trailing 2x2 block
+        const double p = 0.5 * (y - x);
+        const double q = p * p + w;
+        const double z = sqrt(fabs(q));
+        const double xs = x + t;
+        if (q >= 0.) {
+          const double zz = p + (p >= 0. ? z : -z);
+          const double r1 = xs + zz;
+          const double r2 = zz != 0. ? xs - w / zz : r1;
+          emax = fmax(emax, fmax(r1, r2));
+        } else {
+          emax = fmax(emax, xs + p);
+        }
+        nn -= 2;
+        break;
+      }
+      if (total >= total_max) {  // no convergence: bound the rest of the window by its diagonal
+        *failed = 1;
+        for (int i = l; i <= nn; ++i) emax = fmax(emax, a[i * ld + i] + t);
+        nn = l - 1;
+        break;
+      }
+      double xx = x, yy = y, ww = w;
+      if (its > 0 && its % 10 == 0) {  // exceptional shift
+        t += xx;
+        par.each(0, nn + 1, [&](int i) { a[i * ld + i] -= xx; });
+        const double s = fabs(a[nn * ld + nn - 1]) + fabs(a[(nn - 1) * ld + nn - 2]);
+        xx = yy = 0.75 * s;
+        ww = -0.4375 * s * s;
+      }
+      ++its;
+      ++total;
+      // look for two consecutive small subdiagonal elements
+      int m;
+      double p = 0., q = 0., r = 0.;
+      for (m = nn - 2; m >= l; --m) {
+        const double z = a[m * ld + m];
+        const double rr = xx - z, ss = yy - z;
+        p = (rr * ss - ww) / a[(m + 1) * ld + m] + a[m * ld + m + 1];
+        q = a[(m + 1) * ld + m + 1] - z - rr - ss;
+        r = a[(m + 2) * ld + m + 1];
+        const double s = fabs(p) + fabs(q) + fabs(r);
+        p /= s;
+        q /= s;
+        r /= s;
+        if (m == l) break;
+        const double u = fabs(a[m * ld + m - 1]) * (fabs(q) + fabs(r));
+        const double v = fabs(p) * (fabs(a[(m - 1) * ld + m - 1]) + fabs(z) + fabs(a[(m + 1) * ld + m + 1]));
+        if (u + v == v) break;
+      }
+      par.each(m + 2, nn + 1, [&](int i) {
+        a[i * ld + i - 2] = 0.;
+        if (i != m + 2) a[i * ld + i - 3] = 0.;
+      });
+      // the double-shift QR sweep: 3x3 reflectors chase the bulge down the window
+      for (int k = m; k <= nn - 1; ++k) {
+        double xk = 1.;
+        if (k != m) {
+          p = a[k * ld + k - 1];
+          q = a[(k + 1) * ld + k - 1];
+          r = k != nn - 1 ? a[(k + 2) * ld + k - 1] : 0.;
+          xk = fabs(p) + fabs(q) + fabs(r);
+          if (xk != 0.) {
+            p /= xk;
+            q /= xk;
+            r /= xk;
+          }
+        }
+        const double sn = sqrt(p * p + q * q + r * r);
+        const double s = p >= 0. ? sn : -sn;
+        if (s == 0.) continue;
+        const double hx = (p + s) / s, hy = q / s, hz = r / s;
+        const double qq = q / (p + s), rr = r / (p + s);
+        const bool three = k != nn - 1;
+        const double sub = (k == m) ? (l != m ? -a[k * ld + k - 1] : a[k * ld + (k > 0 ? k - 1 : 0)])
+                                    : -s * xk;
+        const bool write_sub = (k != m) || (l != m);
+        // row transformation on rows k..k+2, columns k..nn (and the subdiagonal entry of column k-1)
+        par.each(k, nn + 1, [&](int j) {
+          double pp = a[k * ld + j] + qq * a[(k + 1) * ld + j];
+          if (three) {
+            pp += rr * a[(k + 2) * ld + j];
+            a[(k + 2) * ld + j] -= pp * hz;
+          }
+          a[(k + 1) * ld + j] -= pp * hy;
+          a[k * ld + j] -= pp * hx;
+          if (j == k && write_sub && k > 0) a[k * ld + k - 1] = sub;
+        });
+        // column transformation on columns k..k+2, rows l..min(nn, k+3)
+        const int imax = nn < k + 3 ? nn : k + 3;
+        par.each(l, imax + 1, [&](int i) {
+          double pp = hx * a[i * ld + k] + hy * a[i * ld + k + 1];
+          if (three) {
+            pp += hz * a[i * ld + k + 2];
+            a[i * ld + k + 2] -= pp * rr;
+          }
+          a[i * ld + k + 1] -= pp * qq;
+          a[i * ld + k] -= pp;
+        });
+      }
+    }
+  }
+  return emax;
+}
+
+// Host policy: lane loops run sequentially (used by the CPU unit tests of this file).
+struct HostLanes {
+  template <class F> void each(int lo, int hi, F f) { for (int t = lo; t < hi; ++t) f(t); }
+  template <class F> double sum(int lo, int hi, F f) {
+    double s = 0.;
+    for (int t = lo; t < hi; ++t) s += f(t);
+    return s;
+  }
+};
+
+}  // namespace gfa

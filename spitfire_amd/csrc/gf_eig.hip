@@ -1,0 +1,60 @@
+// Per-block largest-real-part eigenvalue kernel (the pseudo-time limiter input of the flamelet solvers).
+#include "gf_internal.h"
+#include "gf_eig.h"
+#include "../../include/griffon_amd.h"
+
+namespace gfa {
+
+// Device policy for eig_max_real: one 64-lane wave per matrix, lane t owns row/column t.
+struct WaveLanes {
+  int lane;
+  template <class F> __device__ void each(int lo, int hi, F f) {
+    if (lane >= lo && lane < hi) f(lane);
+    __syncthreads();
+  }
+  template <class F> __device__ double sum(int lo, int hi, F f) {
+    double v = (lane >= lo && lane < hi) ? f(lane) : 0.;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);  // identical in every lane
+    return v;
+  }
+};
+
+// expeig for block b: max(0, max Re(lambda(B_b)) - diffterm), written to out[b*n .. b*n+n-1]
+// (flamelet_kernels.cpp:1329-1341). Blocks are column-major n x n at blocks + b*bstride.
+__global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ blocks, long bstride, long nblk, int n,
+                                                    double diffterm, double *__restrict__ out, int *__restrict__ nfail) {
+  extern __shared__ double smem[];
+  const int ld = n | 1;
+  double *a = smem;
+  double *vb = smem + (long)n * ld;
+  WaveLanes par{(int)threadIdx.x};
+  for (long b = blockIdx.x; b < nblk; b += gridDim.x) {
+    const double *src = blocks + b * bstride;
+    for (int e = threadIdx.x; e < n * n; e += 64) a[(e % n) * ld + e / n] = src[e];
+    __syncthreads();
+    int failed = 0;
+    const double em = eig_max_real(par, a, vb, n, ld, &failed);
+    const double ex = fmax(em - diffterm, 0.);
+    for (int q = threadIdx.x; q < n; q += 64) out[b * n + q] = ex;
+    if (failed && threadIdx.x == 0 && nfail) atomicAdd(nfail, 1);
+    __syncthreads();
+  }
+}
+
+size_t expeig_lds_bytes(int n) { return ((size_t)n * (n | 1) + 64) * sizeof(double); }
+
+int launch_expeig(const gf_mech *m, const double *blocks, long bstride, long nblk, int n, double diffterm, double *out,
+                  int *nfail, hipStream_t st) {
+  if (nblk <= 0) return GF_OK;
+  if (n < 1 || n > 64) return fail(GF_ERR_UNSUPPORTED, "eigenvalue blocks must be 1..64 wide");
+  const size_t lds = expeig_lds_bytes(n);
+  // one wave per block; LDS (about 23 KB at n = 53) caps residency near 6 waves per CU, so a grid of
+  // 8 waves per CU keeps every CU busy and lets each wave loop over its share of blocks
+  long grid = (long)(m && m->n_cu > 0 ? m->n_cu : 256) * 8;
+  if (grid > nblk) grid = nblk;
+  hipLaunchKernelGGL(expeig_kernel, dim3((unsigned)grid), dim3(64), lds, st, blocks, bstride, nblk, n, diffterm, out,
+                     nfail);
+  return hipGetLastError() == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "eigenvalue kernel launch failed");
+}
+
+}  // namespace gfa

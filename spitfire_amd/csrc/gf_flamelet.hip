@@ -106,7 +106,10 @@ __global__ void rhs_epilogue_kernel(FlameletIO f, double *out) {
 
 // One thread per entry of the full BTDDOD array: nzi blocks, then (nzi-1)*ns sub, then (nzi-1)*ns sup.
 // The chemistry kernel has already written each point's transformed isobaric block.
-__global__ void jac_epilogue_kernel(MechView mv, FlameletIO f, int scale_and_offset, double prefactor, double *jac) {
+// phase 0: every term; 1: only the heat-loss terms (which the reference adds before its eigenvalue
+// computation); 2: every term but the heat-loss ones
+__global__ void jac_epilogue_kernel(MechView mv, FlameletIO f, int scale_and_offset, double prefactor, double *jac,
+                                    int phase) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int ns = f.ns, nzi = f.nzi;
   const long bsz = (long)ns * ns;
@@ -120,7 +123,7 @@ __global__ void jac_epilogue_kernel(MechView mv, FlameletIO f, int scale_and_off
     const int r = (int)(e % bsz);
     const int col = r / ns, row = r % ns;
     const long pt = iz + 1;
-    if (row == 0 && !f.adiabatic) {  // flamelet_kernels.cpp:1290-1320 on the primitive Jacobian, then transformed
+    if (row == 0 && !f.adiabatic && phase != 2) {  // flamelet_kernels.cpp:1290-1320 on the primitive Jacobian, then transformed
       const double T = f.state[(long)iz * ns];
       const double rho = f.aux[pt * 4 + 0], cp = f.aux[pt * 4 + 1], cps = f.aux[pt * 4 + 2],
                    mmw = f.aux[pt * 4 + 3];
@@ -146,6 +149,10 @@ __global__ void jac_epilogue_kernel(MechView mv, FlameletIO f, int scale_and_off
         v += iC * q * (cpi[ns - 1] - cpi[k]) + (-rho * mmw * mv.dinvm[k]) * drho;
       }
     }
+    if (phase == 1) {
+      jac[e] = v;
+      return;
+    }
     diag = (row == col);
     if (diag) v += f.cmaj[(long)iz * ns + row];  // flamelet_kernels.cpp:1343-1346
     if (row == 0 && col == 0 && f.incl_ef) {      // flamelet_kernels.cpp:1348-1381
@@ -157,6 +164,8 @@ __global__ void jac_epilogue_kernel(MechView mv, FlameletIO f, int scale_and_off
       const double f1 = 0.5 * f.chi[iz] / cp * dTdZ * dcpdZ;
       v -= f1 / cp * cps;
     }
+  } else if (phase == 1) {
+    return;
   } else if (e < nbl + noff) {
     const long q = e - nbl;  // subdiagonal entry (iz-1)*ns + iq gets csub[iz*ns + iq] (flamelet_kernels.cpp:1385-1394)
     v += f.csub[q + ns];
@@ -210,10 +219,10 @@ int launch_flamelet_rhs_epilogue(const FlameletIO &f, double *out, hipStream_t s
 }
 
 int launch_flamelet_jac_epilogue(const gf_mech *m, const FlameletIO &f, int scale_and_offset, double prefactor,
-                                 double *jac, hipStream_t st) {
+                                 double *jac, int phase, hipStream_t st) {
   const long nel = (long)f.ns * (f.nzi * f.ns + 2 * (f.nzi - 1));
   hipLaunchKernelGGL(jac_epilogue_kernel, dim3(nblk(nel, 256)), dim3(256), 0, st, m->view, f, scale_and_offset,
-                     prefactor, jac);
+                     prefactor, jac, phase);
   return hipGetLastError() == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "flamelet jacobian kernel launch failed");
 }
 

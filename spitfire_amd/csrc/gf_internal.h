@@ -160,7 +160,10 @@ struct FlameletIO {
 };
 int launch_flamelet_rhs_epilogue(const FlameletIO &f, double *out, hipStream_t st);
 int launch_flamelet_jac_epilogue(const gf_mech *m, const FlameletIO &f, int scale_and_offset, double prefactor,
-                                 double *jac, hipStream_t st);
+                                 double *jac, int phase, hipStream_t st);
+// per-block largest real eigenvalue part, max(0, max Re(lambda) - diffterm) per block row (gf_eig.hip)
+int launch_expeig(const gf_mech *m, const double *blocks, long bstride, long nblk, int n, double diffterm, double *out,
+                  int *nfail, hipStream_t st);
 int launch_flamelet_stencils(int ns, const double *dz, int nzi, const double *chi, const double *invLe, double *cmaj,
                              double *csub, double *csup, double *mco, double *nco, hipStream_t st);
 int launch_flamelet_jac_indices(int ns, int nzi, int *rows, int *cols, hipStream_t st);

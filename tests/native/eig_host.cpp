@@ -1,0 +1,12 @@
+// CPU build of the device eigenvalue routine (host lane policy), for unit tests only.
+#include <vector>
+#include "../../spitfire_amd/csrc/gf_eig.h"
+
+extern "C" double eig_max_real_host(const double *colmajor, int n, int *failed) {
+  const int ld = n | 1;
+  std::vector<double> a((size_t)n * ld + 1), v(n + 1);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) a[(size_t)i * ld + j] = colmajor[(size_t)j * n + i];
+  gfa::HostLanes par;
+  return gfa::eig_max_real(par, a.data(), v.data(), n, ld, failed);
+}

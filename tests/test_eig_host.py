@@ -1,0 +1,85 @@
+"""The per-block eigenvalue routine of the flamelet pseudo-time limiter (spitfire_amd/csrc/gf_eig.h), built for
+the CPU with its host lane policy, against LAPACK dgeev (numpy.linalg.eigvals): the reference computes the same
+quantity with dgeev at flamelet_kernels.cpp:1329-1341.
+
+Bound: |max Re(lambda) - numpy| <= 1e-12 * max |lambda| (backward-stable eigenvalue algorithms agree to a few
+ulps of the matrix norm; balancing makes that norm the balanced one)."""
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope='module')
+def eiglib():
+    d = tempfile.mkdtemp(prefix='gf_eig_')
+    so = os.path.join(d, 'libeighost.so')
+    subprocess.run(['g++', '-O2', '-std=c++17', '-shared', '-fPIC', '-o', so,
+                    os.path.join(HERE, 'native', 'eig_host.cpp')], check=True)
+    lib = ctypes.CDLL(so)
+    lib.eig_max_real_host.restype = ctypes.c_double
+    lib.eig_max_real_host.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    return lib
+
+
+def _emax(lib, A):
+    A = np.asfortranarray(A, dtype=np.float64)
+    f = ctypes.c_int()
+    r = lib.eig_max_real_host(A.ctypes.data, A.shape[0], ctypes.byref(f))
+    return r, f.value
+
+
+def _check(lib, A, tol=1e-12):
+    r, failed = _emax(lib, A)
+    lam = np.linalg.eigvals(A)
+    assert failed == 0
+    assert abs(r - lam.real.max()) <= tol * max(np.abs(lam).max(), 1e-300), (r, lam.real.max())
+
+
+@pytest.mark.parametrize('n', [1, 2, 3, 4, 7, 10, 31, 53, 63])
+def test_random_matrices(eiglib, n):
+    rng = np.random.default_rng(100 + n)
+    for trial in range(12):
+        A = rng.standard_normal((n, n))
+        if trial % 3 == 1:  # badly scaled rows and columns: balancing matters
+            A *= 10. ** rng.uniform(-4, 4, (n, 1)) * 10. ** rng.uniform(-4, 4, (1, n))
+        _check(eiglib, A)
+
+
+def test_special_structures(eiglib):
+    # rotation blocks (complex pairs), a Jordan-like block, a diagonal matrix, a triangular matrix, zeros
+    R = np.zeros((6, 6))
+    for k, (a, b) in enumerate([(1., 3.), (-2., 0.5), (0.25, 7.)]):
+        R[2 * k:2 * k + 2, 2 * k:2 * k + 2] = [[a, b], [-b, a]]
+    Q = np.linalg.qr(np.random.default_rng(3).standard_normal((6, 6)))[0]
+    _check(eiglib, Q @ R @ Q.T)
+    J = np.diag(np.full(8, 2.)) + np.diag(np.ones(7), 1)
+    _check(eiglib, J, tol=1e-7)  # a defective eigenvalue is only determined to ~eps^(1/8)
+    _check(eiglib, np.diag(np.arange(-5., 5.)))
+    _check(eiglib, np.triu(np.random.default_rng(4).standard_normal((9, 9))))
+    assert _emax(eiglib, np.zeros((5, 5)))[0] == 0.
+    # closed form: companion matrix of (x-1)(x-2)(x+3)(x-0.5) has largest root 2
+    c = np.poly([1., 2., -3., 0.5])
+    C = np.zeros((4, 4))
+    C[0, :] = -c[1:]
+    C[1:, :-1] = np.eye(3)
+    assert abs(_emax(eiglib, C)[0] - 2.) <= 1e-12
+
+
+@pytest.mark.parametrize('fx, mech', [('flamelet_c1.npz', 'mech_c1_data'), ('flamelet_53.npz', 'mech_53_data')])
+def test_flamelet_chemistry_blocks(eiglib, fx, mech, request, oracle_lib):
+    from spitfire_amd.mechdata import CompiledMechanism
+    f = np.load(os.path.join(HERE, 'golden', fx))
+    o = oracle_lib.CpuGriffon(CompiledMechanism(request.getfixturevalue(mech), order='name'))
+    nzi, ns = int(f['nzi']), int(f['ns'])
+    st = o.flamelet_stencils(f['dz'], nzi, f['chi'], np.ones(ns))
+    o.flamelet_jacobian(f['state'], float(f['p']), f['oxy'], f['fuel'], nzi, st, f['chi'], compute_eig=True,
+                        diffterm=20.)
+    pre = o.last_chemistry_blocks
+    for iz in range(nzi):
+        _check(eiglib, pre[iz * ns * ns:(iz + 1) * ns * ns].reshape((ns, ns), order='F'))

@@ -131,3 +131,39 @@ def test_batched_btd_ragged_matches_oracle(mech_53_data, oracle_lib):
         e = np.max(np.abs(Xh[poff[k]:poff[k] + nb * bs] - xr)) / np.max(np.abs(xr))
         say('system', nb, 'blocks: err %.3e' % e)
         assert e <= 1e-11
+
+
+@pytest.mark.parametrize('fx, mech', [('flamelet_c1.npz', 'mech_c1_data'), ('flamelet_53.npz', 'mech_53_data')])
+def test_flamelet_eigenvalue_limiter_matches_reference(fx, mech, request, oracle_lib):
+    """expeig = max(0, max Re(lambda(chemistry block)) - diffterm) per point (flamelet_kernels.cpp:1329-1341):
+    GPU balancing + Hessenberg + Francis QR against the reference's LAPACK dgeev values, to within
+    1e-12 * max|lambda| of each block (eigenvalues agree to rounding of the block norm)."""
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from spitfire_amd.mechdata import CompiledMechanism
+    data = request.getfixturevalue(mech)
+    f = np.load(os.path.join(GOLD, fx))
+    g = ChemicalMechanismSpec(data).griffon
+    nzi, ns = int(f['nzi']), int(f['ns'])
+    nv = nzi * ns
+    cm, cs, cu, mc, nc = [np.zeros(nv) for _ in range(3)] + [np.zeros(nzi), np.zeros(nzi)]
+    g.flamelet_stencils(f['dz'], nzi, f['chi'], np.ones(ns), cm, cs, cu, mc, nc)
+    null = np.zeros(1)
+    nel = ns * (nzi * ns + 2 * (nzi - 1))
+    out, expeig, plain = np.zeros(nel), np.zeros(nv), np.zeros(nel)
+    say('eigenvalue limiter', fx)
+    g.flamelet_jacobian(f['state'], float(f['p']), f['oxy'], f['fuel'], True, null, null, null, null, nzi, cm, cs,
+                        cu, mc, nc, f['chi'], True, 20., False, 0., 0, 0, False, False, False, expeig, out)
+    g.flamelet_jacobian(f['state'], float(f['p']), f['oxy'], f['fuel'], True, null, null, null, null, nzi, cm, cs,
+                        cu, mc, nc, f['chi'], False, 0., False, 0., 0, 0, False, False, False, null, plain)
+    assert np.array_equal(out, plain)  # computing eigenvalues leaves the Jacobian unchanged
+    o = oracle_lib.CpuGriffon(CompiledMechanism(data, order='name'))
+    st = o.flamelet_stencils(f['dz'], nzi, f['chi'], np.ones(ns))
+    o.flamelet_jacobian(f['state'], float(f['p']), f['oxy'], f['fuel'], nzi, st, f['chi'], compute_eig=True,
+                        diffterm=20.)
+    pre = o.last_chemistry_blocks
+    for iz in range(nzi):
+        rad = np.abs(np.linalg.eigvals(pre[iz * ns * ns:(iz + 1) * ns * ns].reshape((ns, ns), order='F'))).max()
+        e = np.abs(expeig[iz * ns:(iz + 1) * ns] - f['expeig'][iz * ns:(iz + 1) * ns]).max()
+        say('  point %d expeig %.6e ref %.6e |diff|/rho %.2e' % (iz, expeig[iz * ns], f['expeig'][iz * ns], e / rad))
+        assert np.all(expeig[iz * ns:(iz + 1) * ns] == expeig[iz * ns])
+        assert e <= 1e-12 * rad

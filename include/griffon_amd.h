@@ -123,6 +123,51 @@ int gf_batch_btddod_full_solve(int nsys, const int *nblocks, int bs, const doubl
                                const double *l_values, const long *loff, const int *pivots, const long *poff,
                                const double *rhs, double *x, const long *xoff, void *stream);
 
+/* ---------------- many steady flamelets at once, device-resident ------------------------------------ */
+/* A batch of adiabatic flamelets sharing one mechanism and pressure, solved together on the GPU by the
+ * reference's pseudo-transient continuation (Flamelet.steady_solve_psitc, flamelet.py:1487-1703): per
+ * flamelet the same Jacobian-refresh rule, eigenvalue-limited local pseudo-time steps, line search and
+ * restarts, with all flamelets advancing in lock step and every refresh, factorization, solve and RHS
+ * evaluation batched over the flamelets that need it. Replaces a loop of Flamelet.compute_steady_state
+ * calls over the flamelets of a library (tabulation.py, one flamelet per dissipation rate).
+ *
+ * Flamelet f has nzi[f] >= 2 interior points; dz holds its nzi[f]+1 grid spacings and chi its nzi[f]+2
+ * dissipation rates (full grid, flamelet.py:458-466), concatenated over f; oxy/fuel hold the ns-long
+ * boundary states [T, Y_0..Y_{ns-2}] of every flamelet. Unity Lewis numbers. Host arrays. */
+typedef struct gf_flamelet_batch gf_flamelet_batch;
+typedef struct gf_psitc_options {
+  double tolerance;               /* 1e-6   */
+  int max_iterations;             /* 400    */
+  double ds_init;                 /* 1      */
+  double ds_init_decrease;        /* 4      */
+  int adaptive_restart;           /* 1      */
+  int max_recursion_depth;        /* 20     */
+  double diffusion_factor;        /* 4      */
+  int global_ds;                  /* 0      */
+  double ds_safety;               /* 0.1    */
+  double ds_ramp;                 /* 1.1    */
+  double ds_max;                  /* 1e4    */
+  double max_factor_line_search;  /* 1.5    */
+  double max_allowed_residual;    /* 1e6    */
+  double min_allowable_state_var; /* -1e-6  */
+  int lu_precision;               /* 64: FP64 block LU; 32: FP32 block LU factors (FP64 residuals) */
+} gf_psitc_options;
+void gf_psitc_options_default(gf_psitc_options *opts);
+int gf_flamelet_batch_create(gf_mech *m, int nfl, const int *nzi, double pressure, const double *dz,
+                             const double *chi, const double *oxy, const double *fuel, int include_enthalpy_flux,
+                             int include_variable_cp, gf_flamelet_batch **out);
+int gf_flamelet_batch_destroy(gf_flamelet_batch *b);
+/* interior states, point-major per flamelet, concatenated: sum(nzi)*ns doubles; set_state also sets the
+ * initial guess the restarts return to */
+int gf_flamelet_batch_set_state(gf_flamelet_batch *b, const double *interior);
+int gf_flamelet_batch_get_state(gf_flamelet_batch *b, double *interior);
+/* flamelet RHS of every flamelet at the current state (same layout as the state) */
+int gf_flamelet_batch_rhs(gf_flamelet_batch *b, double *out);
+/* iterations/converged/residual: per flamelet (may be null); counters (may be null): [0] iterations summed
+ * over flamelets, [1] Jacobian refreshes, [2] RHS evaluations, [3] synchronizing solver steps */
+int gf_flamelet_batch_steady_psitc(gf_flamelet_batch *b, const gf_psitc_options *opts, int *iterations,
+                                   int *converged, double *residual, long long *counters);
+
 #ifdef __cplusplus
 }
 #endif

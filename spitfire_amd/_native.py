@@ -50,7 +50,24 @@ SIGNATURES = {
     'gf_batch_reactor_jac_isobaric': [_V, _int, _V, _long, _long, _V, _long, _V, _long, _long, _V, _long, _V],
     'gf_batch_btddod_full_factorize': [_int, _V, _int, _V, _V, _V, _V, _V, _V, _V],
     'gf_batch_btddod_full_solve': [_int, _V, _int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V],
+    'gf_psitc_options_default': [_V],
+    'gf_flamelet_batch_create': [_V, _int, _I, _dbl, _D, _D, _D, _D, _int, _int, ctypes.POINTER(_V)],
+    'gf_flamelet_batch_destroy': [_V],
+    'gf_flamelet_batch_set_state': [_V, _D],
+    'gf_flamelet_batch_get_state': [_V, _D],
+    'gf_flamelet_batch_rhs': [_V, _D],
+    'gf_flamelet_batch_steady_psitc': [_V, _V, _I, _I, _D, ctypes.POINTER(ctypes.c_longlong)],
 }
+
+
+class PsitcOptions(ctypes.Structure):
+    """gf_psitc_options (include/griffon_amd.h)"""
+    _fields_ = [('tolerance', _dbl), ('max_iterations', _int), ('ds_init', _dbl), ('ds_init_decrease', _dbl),
+                ('adaptive_restart', _int), ('max_recursion_depth', _int), ('diffusion_factor', _dbl),
+                ('global_ds', _int), ('ds_safety', _dbl), ('ds_ramp', _dbl), ('ds_max', _dbl),
+                ('max_factor_line_search', _dbl), ('max_allowed_residual', _dbl), ('min_allowable_state_var', _dbl),
+                ('lu_precision', _int)]
+
 
 _lib = None
 
@@ -78,7 +95,7 @@ def lib():
         for name, args in SIGNATURES.items():
             f = getattr(l, name)
             f.argtypes = args
-            f.restype = ctypes.c_char_p if name == 'gf_last_error' else ctypes.c_int
+            f.restype = {'gf_last_error': ctypes.c_char_p, 'gf_psitc_options_default': None}.get(name, ctypes.c_int)
         _lib = l
     return _lib
 

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIBDIR = os.path.join(HERE, '_lib')
 LIB = os.path.join(LIBDIR, 'libgriffon_amd.so')
-SOURCES = ['gf_mech.cpp', 'gf_capi.cpp', 'gf_chem.hip', 'gf_btd.hip', 'gf_flamelet.hip', 'gf_eig.hip']
+SOURCES = ['gf_mech.cpp', 'gf_capi.cpp', 'gf_chem.hip', 'gf_btd.hip', 'gf_flamelet.hip', 'gf_eig.hip', 'gf_fbatch.cpp']
 ARCH = os.environ.get('GF_OFFLOAD_ARCH', 'gfx950')
 
 

@@ -151,10 +151,11 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
 
     // ---------------- P0: mixture thermo (wave per state, lane = species) ----------------
     if (wave < S) {
-      const long i = first + wave;
+      const long il = first + wave;
       double *sp = sbase + wave * stride;
       double *sc = sp + NSA * SP_STRIDE;
-      if (i < a.n) {
+      if (il < a.n) {
+        const long i = a.sel ? (long)a.sel[il] : il;
         const double *xi = a.x + i * a.xs_i;
         const double T = xi[0];
         double yk = 0.;
@@ -383,12 +384,13 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
 
     // ---------------- P2: species gathers + reductions (wave per state, lane = species) ----------------
     if (wave < S) {
-      const long i = first + wave;
+      const long il = first + wave;
       double *sp = sbase + wave * stride;
       double *sc = sp + NSA * SP_STRIDE;
       const double *rq = sc + N_SCAL;
       const double *v = rq + 5 * nr;
-      if (i < a.n) {
+      if (il < a.n) {
+        const long i = a.sel ? (long)a.sel[il] : il;
         double w = 0., w0 = 0., w1 = 0., A = 0., B = 0., DG = 0.;
         if (lane < ns) {
           const double Mk = mv.mw[lane];
@@ -444,8 +446,17 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
       // ---------------- P3: Jacobian columns (wave = column, lane = row; lane ns = the last species) -------
       int nact = 0;
       double iR[S], iC[S], iRC[S], roT[S], rhs0[S], rmw[S], T00[S];
+      double *jb[S];  // Jacobian block of each state (nullptr: none)
       for (int s = 0; s < S; ++s) {
-        if (first + s < a.n) nact = s + 1;
+        jb[s] = nullptr;
+        if (first + s < a.n) {
+          nact = s + 1;
+          const long q = a.sel ? (long)a.sel[first + s] : first + s;
+          if (a.jac) {
+            const long off = a.js_off ? a.js_off[q] : q * a.js_i;
+            if (off >= 0) jb[s] = a.jac + off;
+          }
+        }
         const double *sc = sbase + s * stride + NSA * SP_STRIDE;
         iR[s] = 1. / sc[S_RHO];
         iC[s] = 1. / sc[S_CP];
@@ -499,7 +510,8 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
             }
           }
           if (lane < ns)
-            for (int s = 0; s < nact; ++s) a.jac[(first + s) * a.js_i + (long)c * ns + lane] = val[s];
+            for (int s = 0; s < nact; ++s)
+              if (jb[s]) jb[s][(long)c * ns + lane] = val[s];
         }
       }
       if (a.sens) {
@@ -507,7 +519,8 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
         for (int s = 0; s < nact; ++s) {
           const double *sp = sbase + s * stride;
           const double *v = sp + NSA * SP_STRIDE + N_SCAL + 5 * nr;
-          double *so = a.sens + (first + s) * (long)(ns + 1) * (ns + 1);
+          const long q = a.sel ? (long)a.sel[first + s] : first + s;
+          double *so = a.sens + q * (long)(ns + 1) * (ns + 1);
           for (int e = threadIdx.x; e < (ns + 1) * (ns + 1); e += THREADS) {
             const int c = e / (ns + 1), k = e - c * (ns + 1);
             double val = 0.;

@@ -21,21 +21,25 @@ struct WaveLanes {
 
 // expeig for block b: max(0, max Re(lambda(B_b)) - diffterm), written to out[b*n .. b*n+n-1]
 // (flamelet_kernels.cpp:1329-1341). Blocks are column-major n x n at blocks + b*bstride.
-__global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ blocks, long bstride, long nblk, int n,
-                                                    double diffterm, double *__restrict__ out, int *__restrict__ nfail) {
+// With boff/ooff given, block b starts at blocks + boff[b] and its n outputs at out + ooff[b].
+__global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ blocks, long bstride,
+                                                    const long *__restrict__ boff, const long *__restrict__ ooff,
+                                                    long nblk, int n, double diffterm, double *__restrict__ out,
+                                                    int *__restrict__ nfail) {
   extern __shared__ double smem[];
   const int ld = n | 1;
   double *a = smem;
   double *vb = smem + (long)n * ld;
   WaveLanes par{(int)threadIdx.x};
   for (long b = blockIdx.x; b < nblk; b += gridDim.x) {
-    const double *src = blocks + b * bstride;
+    const double *src = blocks + (boff ? boff[b] : b * bstride);
     for (int e = threadIdx.x; e < n * n; e += 64) a[(e % n) * ld + e / n] = src[e];
     __syncthreads();
     int failed = 0;
     const double em = eig_max_real(par, a, vb, n, ld, &failed);
     const double ex = fmax(em - diffterm, 0.);
-    for (int q = threadIdx.x; q < n; q += 64) out[b * n + q] = ex;
+    double *dst = out + (ooff ? ooff[b] : b * n);
+    for (int q = threadIdx.x; q < n; q += 64) dst[q] = ex;
     if (failed && threadIdx.x == 0 && nfail) atomicAdd(nfail, 1);
     __syncthreads();
   }
@@ -43,8 +47,8 @@ __global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ b
 
 size_t expeig_lds_bytes(int n) { return ((size_t)n * (n | 1) + 64) * sizeof(double); }
 
-int launch_expeig(const gf_mech *m, const double *blocks, long bstride, long nblk, int n, double diffterm, double *out,
-                  int *nfail, hipStream_t st) {
+int launch_expeig_list(const gf_mech *m, const double *blocks, const long *boff, const long *ooff, long nblk, int n,
+                       double diffterm, double *out, int *nfail, hipStream_t st) {
   if (nblk <= 0) return GF_OK;
   if (n < 1 || n > 64) return fail(GF_ERR_UNSUPPORTED, "eigenvalue blocks must be 1..64 wide");
   const size_t lds = expeig_lds_bytes(n);
@@ -52,9 +56,15 @@ int launch_expeig(const gf_mech *m, const double *blocks, long bstride, long nbl
   // 8 waves per CU keeps every CU busy and lets each wave loop over its share of blocks
   long grid = (long)(m && m->n_cu > 0 ? m->n_cu : 256) * 8;
   if (grid > nblk) grid = nblk;
-  hipLaunchKernelGGL(expeig_kernel, dim3((unsigned)grid), dim3(64), lds, st, blocks, bstride, nblk, n, diffterm, out,
-                     nfail);
+  hipLaunchKernelGGL(expeig_kernel, dim3((unsigned)grid), dim3(64), lds, st, blocks, (long)n * n, boff, ooff, nblk, n,
+                     diffterm, out, nfail);
   return hipGetLastError() == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "eigenvalue kernel launch failed");
+}
+
+int launch_expeig(const gf_mech *m, const double *blocks, long bstride, long nblk, int n, double diffterm, double *out,
+                  int *nfail, hipStream_t st) {
+  if (bstride != (long)n * n) return fail(GF_ERR_ARG, "eigenvalue blocks must be contiguous");
+  return launch_expeig_list(m, blocks, nullptr, nullptr, nblk, n, diffterm, out, nfail, st);
 }
 
 }  // namespace gfa

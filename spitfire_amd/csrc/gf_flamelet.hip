@@ -58,11 +58,9 @@ __device__ __forceinline__ double max_interior_T(const FlameletIO &f) {
   return m;
 }
 
-// one thread per interior (point, variable)
-__global__ void rhs_epilogue_kernel(FlameletIO f, double *out) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// flamelet RHS entry e = (interior point, variable)
+__device__ void rhs_entry(const FlameletIO &f, long e, double *out) {
   const int ns = f.ns, nzi = f.nzi;
-  if (e >= (long)nzi * ns) return;
   const int i = (int)(e / ns), l = (int)(e % ns);
   const long pt = i + 1;  // chemistry point index
   const double *sm = (i == 0) ? f.oxy : f.state + (long)(i - 1) * ns;
@@ -104,20 +102,26 @@ __global__ void rhs_epilogue_kernel(FlameletIO f, double *out) {
   out[e] = v;
 }
 
-// One thread per entry of the full BTDDOD array: nzi blocks, then (nzi-1)*ns sub, then (nzi-1)*ns sup.
-// The chemistry kernel has already written each point's transformed isobaric block.
+// one thread per interior (point, variable)
+__global__ void rhs_epilogue_kernel(FlameletIO f, double *out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)f.nzi * f.ns) return;
+  rhs_entry(f, e, out);
+}
+
+// Entry e of the flamelet Jacobian (BTDDOD layout: nzi blocks, then (nzi-1)*ns sub, then (nzi-1)*ns sup) from v, the value the chemistry kernel left there (the off-diagonal entries
+// accumulate onto v). Returns false when the entry is left unchanged (phase 1 outside the heat-loss rows).
+// diag_row is the row of a block-diagonal entry on the diagonal, else -1.
 // phase 0: every term; 1: only the heat-loss terms (which the reference adds before its eigenvalue
 // computation); 2: every term but the heat-loss ones
-__global__ void jac_epilogue_kernel(MechView mv, FlameletIO f, int scale_and_offset, double prefactor, double *jac,
-                                    int phase) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ bool jac_entry(const MechView &mv, const FlameletIO &f, long e, int scale_and_offset, double prefactor,
+                          int phase, double &v, int &diag_row) {
   const int ns = f.ns, nzi = f.nzi;
   const long bsz = (long)ns * ns;
   const long nbl = (long)nzi * bsz;
   const long noff = (long)(nzi - 1) * ns;
-  if (e >= nbl + 2 * noff) return;
-  double v = jac[e];
   bool diag = false;
+  diag_row = -1;
   if (e < nbl) {
     const int iz = (int)(e / bsz);
     const int r = (int)(e % bsz);
@@ -149,11 +153,9 @@ __global__ void jac_epilogue_kernel(MechView mv, FlameletIO f, int scale_and_off
         v += iC * q * (cpi[ns - 1] - cpi[k]) + (-rho * mmw * mv.dinvm[k]) * drho;
       }
     }
-    if (phase == 1) {
-      jac[e] = v;
-      return;
-    }
+    if (phase == 1) return true;
     diag = (row == col);
+    if (diag) diag_row = iz * ns + row;
     if (diag) v += f.cmaj[(long)iz * ns + row];  // flamelet_kernels.cpp:1343-1346
     if (row == 0 && col == 0 && f.incl_ef) {      // flamelet_kernels.cpp:1348-1381
       const double cp = f.aux[pt * 4 + 1], cps = f.aux[pt * 4 + 2];
@@ -165,7 +167,7 @@ __global__ void jac_epilogue_kernel(MechView mv, FlameletIO f, int scale_and_off
       v -= f1 / cp * cps;
     }
   } else if (phase == 1) {
-    return;
+    return false;
   } else if (e < nbl + noff) {
     const long q = e - nbl;  // subdiagonal entry (iz-1)*ns + iq gets csub[iz*ns + iq] (flamelet_kernels.cpp:1385-1394)
     v += f.csub[q + ns];
@@ -177,7 +179,18 @@ __global__ void jac_epilogue_kernel(MechView mv, FlameletIO f, int scale_and_off
     v *= prefactor;
     if (diag) v -= 1.;
   }
-  jac[e] = v;
+  return true;
+}
+
+// One thread per entry of the full BTDDOD array: nzi blocks, then (nzi-1)*ns sub, then (nzi-1)*ns sup.
+// The chemistry kernel has already written each point's transformed isobaric block.
+__global__ void jac_epilogue_kernel(MechView mv, FlameletIO f, int scale_and_offset, double prefactor, double *jac,
+                                    int phase) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)f.ns * (f.nzi * f.ns + 2 * (f.nzi - 1))) return;
+  double v = jac[e];
+  int dr;
+  if (jac_entry(mv, f, e, scale_and_offset, prefactor, phase, v, dr)) jac[e] = v;
 }
 
 __global__ void scale_add_diag_kernel(double *val, double a, const double *dg, double b, int nb, int bs) {
@@ -192,6 +205,197 @@ __global__ void scale_add_diag_kernel(double *val, double a, const double *dg, d
     if (r / bs == r % bs) v += b * dg[(long)iz * bs + r % bs];
   }
   val[e] = v;
+}
+
+// ---------------- many flamelets at once ----------------
+// flamelet owning global index e: the last f with off[f] <= e
+__device__ __forceinline__ int fb_find(const long *off, int B, long e) {
+  int lo = 0, hi = B - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ FlameletIO fb_io(const FlameletBatchView &b, int f, const double *X) {
+  FlameletIO io;
+  const int nzi = b.nzi[f], ns = b.ns;
+  const long p0 = b.pt0[f], d0 = b.dof0[f], i0 = b.ipt0[f];
+  io.nzi = nzi;
+  io.ns = ns;
+  io.state = X + (p0 + 1) * ns;
+  io.oxy = X + p0 * ns;
+  io.fuel = X + (p0 + nzi + 1) * ns;
+  io.cmaj = b.cmaj + d0;
+  io.csub = b.csub + d0;
+  io.csup = b.csup + d0;
+  io.mco = b.mco + i0;
+  io.nco = b.nco + i0;
+  io.chi = b.chi + b.chi0[f];
+  io.adiabatic = 1;
+  io.incl_ef = b.incl_ef;
+  io.incl_vcp = b.incl_vcp;
+  io.chem_rhs = b.chem_rhs + p0 * ns;
+  io.aux = b.aux + p0 * 4;
+  io.cpi = b.cpi + p0 * ns;
+  return io;
+}
+
+// mixture-fraction stencils of every flamelet (unity Lewis numbers), dz of flamelet f at dz + dz0[f]
+__global__ void fb_stencils_kernel(FlameletBatchView b, const double *dz, const long *dz0, double *cmaj, double *csub,
+                                   double *csup, double *mco, double *nco) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= b.NV) return;
+  const int f = fb_find(b.dof0, b.B, e);
+  const long le = e - b.dof0[f];
+  const int i = (int)(le / b.ns), l = (int)(le % b.ns);
+  const double *d = dz + dz0[f];
+  const double *chi = b.chi + b.chi0[f];
+  const double dzt = d[i] + d[i + 1];
+  cmaj[e] = -chi[1 + i] / (d[i] * d[i + 1]);
+  csub[e] = chi[1 + i] / (dzt * d[i]);
+  csup[e] = chi[1 + i] / (dzt * d[i + 1]);
+  if (l == 0) {
+    nco[b.ipt0[f] + i] = 1 / (d[i] + d[i + 1]);
+    mco[b.ipt0[f] + i] = -nco[b.ipt0[f] + i];
+  }
+}
+
+__global__ void fb_rhs_kernel(FlameletBatchView b, const double *X, const int *flag, double *R) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= b.NV) return;
+  const int f = fb_find(b.dof0, b.B, e);
+  if (flag && !flag[f]) return;
+  const FlameletIO io = fb_io(b, f, X);
+  rhs_entry(io, e - b.dof0[f], R + b.dof0[f]);
+}
+
+// the pseudo-transient matrix diag(1/ds) - J of each flagged flamelet (flamelet.py:1597-1606: a fresh
+// Jacobian, then btddod_scale_and_add_diagonal(J, -1, 1/ds, 1)); the chemistry blocks are already in J
+__global__ void fb_jac_psitc_kernel(MechView mv, FlameletBatchView b, const double *X, const int *flag,
+                                    const double *ds, double *J) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= b.NJ) return;
+  const int f = fb_find(b.voff, b.B, e);
+  if (!flag[f]) return;
+  const FlameletIO io = fb_io(b, f, X);
+  const long el = e - b.voff[f];
+  double v = el < (long)io.nzi * io.ns * io.ns ? J[e] : 0.;
+  int dr;
+  jac_entry(mv, io, el, 0, 0., 0, v, dr);
+  double out = -v;
+  if (dr >= 0) out += 1. / ds[b.dof0[f] + dr];
+  J[e] = out;
+}
+
+// per flagged flamelet: max |R * 1/scale| (T scaled by 1e3, flamelet.py variable scales; NaN if any entry is
+// not finite) and the smallest interior state value
+__global__ void __launch_bounds__(256) fb_norm_kernel(FlameletBatchView b, const int *flag, const double *R,
+                                                      const double *X, double *res, double *xmin) {
+  const int f = blockIdx.x;
+  if (flag && !flag[f]) return;
+  const long n = (long)b.nzi[f] * b.ns, d0 = b.dof0[f];
+  const double *x = X ? X + (b.pt0[f] + 1) * b.ns : nullptr;
+  double m = 0., mn = INFINITY;
+  int bad = 0;
+  for (long e = threadIdx.x; e < n; e += 256) {
+    const double r = R[d0 + e];
+    const double a = fabs(r * ((e % b.ns) == 0 ? 1. / 1.e3 : 1.));
+    if (!isfinite(r)) bad = 1;
+    m = fmax(m, a);
+    if (x) {
+      const double xv = x[e];
+      if (!isfinite(xv)) bad = 1;
+      mn = fmin(mn, xv);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    m = fmax(m, __shfl_xor(m, o, 64));
+    mn = fmin(mn, __shfl_xor(mn, o, 64));
+    bad |= __shfl_xor(bad, o, 64);
+  }
+  __shared__ double sm[4], smn[4];
+  __shared__ int sb[4];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sm[w] = m; smn[w] = mn; sb[w] = bad; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) { m = fmax(m, sm[k]); mn = fmin(mn, smn[k]); bad |= sb[k]; }
+    res[f] = bad ? NAN : m;
+    if (xmin) xmin[f] = bad ? NAN : mn;
+  }
+}
+
+// Xt = X + dX over the interior of each flagged flamelet, first scaling dX by scale[f] when given
+__global__ void fb_trial_kernel(FlameletBatchView b, const int *flag, const double *scale, const double *X,
+                                double *Xt, double *dX) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= b.NV) return;
+  const int f = fb_find(b.dof0, b.B, e);
+  if (!flag[f]) return;
+  double d = dX[e];
+  if (scale) {
+    d *= scale[f];
+    dX[e] = d;
+  }
+  const long q = (b.pt0[f] + 1) * b.ns + (e - b.dof0[f]);
+  Xt[q] = X[q] + d;
+}
+
+__global__ void fb_copy_interior_kernel(FlameletBatchView b, const int *flag, const double *src, double *dst) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= b.NV) return;
+  const int f = fb_find(b.dof0, b.B, e);
+  if (flag && !flag[f]) return;
+  const long q = (b.pt0[f] + 1) * b.ns + (e - b.dof0[f]);
+  dst[q] = src[q];
+}
+
+__global__ void fb_copy_dofs_kernel(FlameletBatchView b, const int *flag, const double *src, double *dst) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= b.NV) return;
+  const int f = fb_find(b.dof0, b.B, e);
+  if (flag && !flag[f]) return;
+  dst[e] = src[e];
+}
+
+__global__ void fb_fill_dofs_kernel(FlameletBatchView b, const int *flag, const double *value, double *dst) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= b.NV) return;
+  const int f = fb_find(b.dof0, b.B, e);
+  if (flag && !flag[f]) return;
+  dst[e] = value[f];
+}
+
+// ds <- min(safety / (expeig + 1e-16), ramp * ds, dsmax), then the flamelet minimum everywhere when
+// global_min[f] (flamelet.py:1599-1604); expeig = max(0, eig - diffterm[f]) from eig = max(0, max Re(lambda))
+__global__ void __launch_bounds__(256) fb_ds_kernel(FlameletBatchView b, const int *flag, const int *global_min,
+                                                    const double *eig, const double *diffterm, double safety,
+                                                    double ramp, double dsmax, double *ds) {
+  const int f = blockIdx.x;
+  if (!flag[f]) return;
+  const long n = (long)b.nzi[f] * b.ns, d0 = b.dof0[f];
+  double mn = INFINITY;
+  for (long e = threadIdx.x; e < n; e += 256) {
+    const double ex = fmax(eig[d0 + e] - diffterm[f], 0.);
+    const double v = fmin(fmin(safety / (ex + 1.e-16), ramp * ds[d0 + e]), dsmax);
+    ds[d0 + e] = v;
+    mn = fmin(mn, v);
+  }
+  if (!global_min[f]) return;
+  for (int o = 32; o > 0; o >>= 1) mn = fmin(mn, __shfl_xor(mn, o, 64));
+  __shared__ double smn[4];
+  if ((threadIdx.x & 63) == 0) smn[threadIdx.x >> 6] = mn;
+  __syncthreads();
+  mn = fmin(fmin(smn[0], smn[1]), fmin(smn[2], smn[3]));
+  for (long e = threadIdx.x; e < n; e += 256) ds[d0 + e] = mn;
+}
+
+__global__ void fb_clip_kernel(long n, double *X) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n && X[e] < 0.) X[e] = 0.;
 }
 
 inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -231,5 +435,53 @@ int launch_scale_and_add_diagonal(double *v, double a, const double *dg, double 
   hipLaunchKernelGGL(scale_add_diag_kernel, dim3(nblk(nel, 256)), dim3(256), 0, st, v, a, dg, b, nb, bs);
   return hipGetLastError() == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "scale_and_add_diagonal launch failed");
 }
+
+#define FB_LAUNCH(name, n, ...)                                                                 \
+  do {                                                                                          \
+    if ((n) > 0) hipLaunchKernelGGL(name, dim3(nblk((n), 256)), dim3(256), 0, st, __VA_ARGS__); \
+    if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, #name " launch failed");        \
+    return GF_OK;                                                                               \
+  } while (0)
+
+int launch_fb_stencils(const FlameletBatchView &b, const double *dz, const long *dz0, double *cmaj, double *csub,
+                       double *csup, double *mco, double *nco, hipStream_t st) {
+  FB_LAUNCH(fb_stencils_kernel, b.NV, b, dz, dz0, cmaj, csub, csup, mco, nco);
+}
+int launch_fb_rhs(const FlameletBatchView &b, const double *X, const int *flag, double *R, hipStream_t st) {
+  FB_LAUNCH(fb_rhs_kernel, b.NV, b, X, flag, R);
+}
+int launch_fb_jac_psitc(const gf_mech *m, const FlameletBatchView &b, const double *X, const int *flag,
+                        const double *ds, double *J, hipStream_t st) {
+  FB_LAUNCH(fb_jac_psitc_kernel, b.NJ, m->view, b, X, flag, ds, J);
+}
+int launch_fb_trial(const FlameletBatchView &b, const int *flag, const double *scale, const double *X, double *Xt,
+                    double *dX, hipStream_t st) {
+  FB_LAUNCH(fb_trial_kernel, b.NV, b, flag, scale, X, Xt, dX);
+}
+int launch_fb_copy_interior(const FlameletBatchView &b, const int *flag, const double *src, double *dst,
+                            hipStream_t st) {
+  FB_LAUNCH(fb_copy_interior_kernel, b.NV, b, flag, src, dst);
+}
+int launch_fb_copy_dofs(const FlameletBatchView &b, const int *flag, const double *src, double *dst, hipStream_t st) {
+  FB_LAUNCH(fb_copy_dofs_kernel, b.NV, b, flag, src, dst);
+}
+int launch_fb_fill_dofs(const FlameletBatchView &b, const int *flag, const double *value, double *dst,
+                        hipStream_t st) {
+  FB_LAUNCH(fb_fill_dofs_kernel, b.NV, b, flag, value, dst);
+}
+int launch_fb_norms(const FlameletBatchView &b, const int *flag, const double *R, const double *X, double *res,
+                    double *xmin, hipStream_t st) {
+  if (b.B <= 0) return GF_OK;
+  hipLaunchKernelGGL(fb_norm_kernel, dim3(b.B), dim3(256), 0, st, b, flag, R, X, res, xmin);
+  return hipGetLastError() == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "fb_norm_kernel launch failed");
+}
+int launch_fb_ds(const FlameletBatchView &b, const int *flag, const int *global_min, const double *expeig,
+                 const double *diffterm, double safety, double ramp, double dsmax, double *ds, hipStream_t st) {
+  if (b.B <= 0) return GF_OK;
+  hipLaunchKernelGGL(fb_ds_kernel, dim3(b.B), dim3(256), 0, st, b, flag, global_min, expeig, diffterm, safety, ramp,
+                     dsmax, ds);
+  return hipGetLastError() == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "fb_ds_kernel launch failed");
+}
+int launch_fb_clip_points(long n, double *X, hipStream_t st) { FB_LAUNCH(fb_clip_kernel, n, n, X); }
 
 }  // namespace gfa

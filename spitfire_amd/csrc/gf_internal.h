@@ -141,6 +141,10 @@ struct ChemIO {
   double *sens = nullptr;  // raw option-0 sensitivities [n][(ns+1)^2]
   double *aux = nullptr;   // per state {rho, cp, dcp/dT, mixture molecular weight}
   double *cpi = nullptr;   // per state species heat capacities [n][ns]
+  // optional: logical state i reads and writes physical row sel[i] of every array above
+  const int *sel = nullptr;
+  // optional: the Jacobian block of physical row q starts at jac + js_off[q] (no block if negative)
+  const long *js_off = nullptr;
 };
 int launch_chem(const gf_mech *m, const ChemIO &io, hipStream_t st);
 
@@ -168,6 +172,38 @@ int launch_flamelet_stencils(int ns, const double *dz, int nzi, const double *ch
                              double *csub, double *csup, double *mco, double *nco, hipStream_t st);
 int launch_flamelet_jac_indices(int ns, int nzi, int *rows, int *cols, hipStream_t st);
 int launch_scale_and_add_diagonal(double *v, double a, const double *dg, double b, int nb, int bs, hipStream_t st);
+// Many flamelets at once (gf_flamelet.hip kernels, gf_fbatch.cpp driver). Flamelet f has nzi[f] interior
+// points; its points [oxidizer, interior..., fuel] are rows pt0[f] .. pt0[f]+nzi[f]+1 of the point arrays
+// (X, chem_rhs, cpi: ns per point; aux: 4 per point); its interior unknowns start at dof0[f] in the dof
+// arrays (cmaj, csub, csup, R, dX, ds, expeig), its interior points at ipt0[f] (mco, nco), its full grid at
+// chi0[f] (chi), and its BTDDOD Jacobian at voff[f].
+struct FlameletBatchView {
+  int B = 0, ns = 0, incl_ef = 0, incl_vcp = 0;
+  const int *nzi = nullptr;
+  const long *pt0 = nullptr, *dof0 = nullptr, *ipt0 = nullptr, *chi0 = nullptr, *voff = nullptr;
+  const double *cmaj = nullptr, *csub = nullptr, *csup = nullptr, *mco = nullptr, *nco = nullptr, *chi = nullptr;
+  const double *chem_rhs = nullptr, *aux = nullptr, *cpi = nullptr;
+  long NV = 0, NJ = 0;  // total unknowns, total Jacobian entries
+};
+int launch_fb_stencils(const FlameletBatchView &b, const double *dz, const long *dz0, double *cmaj, double *csub,
+                       double *csup, double *mco, double *nco, hipStream_t st);
+int launch_fb_rhs(const FlameletBatchView &b, const double *X, const int *flag, double *R, hipStream_t st);
+int launch_fb_jac_psitc(const gf_mech *m, const FlameletBatchView &b, const double *X, const int *flag,
+                        const double *ds, double *J, hipStream_t st);
+int launch_fb_norms(const FlameletBatchView &b, const int *flag, const double *R, const double *X, double *res,
+                    double *xmin, hipStream_t st);
+int launch_fb_trial(const FlameletBatchView &b, const int *flag, const double *scale, const double *X, double *Xt,
+                    double *dX, hipStream_t st);
+int launch_fb_copy_interior(const FlameletBatchView &b, const int *flag, const double *src, double *dst,
+                            hipStream_t st);
+int launch_fb_copy_dofs(const FlameletBatchView &b, const int *flag, const double *src, double *dst, hipStream_t st);
+int launch_fb_ds(const FlameletBatchView &b, const int *flag, const int *global_min, const double *expeig,
+                 const double *diffterm, double safety, double ramp, double dsmax, double *ds, hipStream_t st);
+int launch_fb_fill_dofs(const FlameletBatchView &b, const int *flag, const double *value, double *dst,
+                        hipStream_t st);
+int launch_fb_clip_points(long n, double *X, hipStream_t st);
+int launch_expeig_list(const gf_mech *m, const double *blocks, const long *boff, const long *ooff, long nblk, int n,
+                       double diffterm, double *out, int *nfail, hipStream_t st);
 // block-tridiagonal kernels (gf_btd.hip)
 int launch_btd_factor(int nsys, const int *nblocks, int bs, double *values, const long *voff, double *l_values,
                       const long *loff, int *pivots, const long *poff, hipStream_t st);

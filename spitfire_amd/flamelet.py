@@ -398,3 +398,124 @@ class Flamelet:
     @property
     def pressure(self):
         return self._pressure
+
+
+class FlameletBatch:
+    """Many adiabatic flamelets of one mechanism and pressure solved together on the GPU.
+
+    The flamelets keep their own grids, dissipation rates, boundary streams and initial conditions; the
+    steady solve is the reference's pseudo-transient continuation (Flamelet.steady_solve_psitc,
+    flamelet.py:1487-1703) applied to every flamelet with device-resident state and batched kernels
+    (gf_flamelet_batch_* in include/griffon_amd.h). Converged states are written back to each Flamelet
+    (current_interior_state), as compute_steady_state does for one flamelet.
+    """
+
+    def __init__(self, flamelets):
+        from ._native import lib, check, dptr, iptr
+        import ctypes
+        self._flamelets = list(flamelets)
+        if not self._flamelets:
+            raise ValueError('empty flamelet batch')
+        f0 = self._flamelets[0]
+        mech = f0._mechanism
+        for f in self._flamelets:
+            if f._mechanism is not mech or f._pressure != f0._pressure:
+                raise ValueError('flamelets in a batch share one mechanism and one pressure')
+            if (f._include_enthalpy_flux, f._include_variable_cp) != (f0._include_enthalpy_flux,
+                                                                      f0._include_variable_cp):
+                raise ValueError('flamelets in a batch share the enthalpy-flux and variable-cp options')
+        self._lib = lib()
+        self._mech = mech
+        self._ns = mech.n_species
+        self._nzi = np.array([f._nz_interior for f in self._flamelets], dtype=np.int32)
+        self._offsets = np.concatenate([[0], np.cumsum(self._nzi.astype(np.int64) * self._ns)])
+        dz = np.ascontiguousarray(np.concatenate([f._dz for f in self._flamelets]))
+        chi = np.ascontiguousarray(np.concatenate([f._x for f in self._flamelets]))
+        oxy = np.ascontiguousarray(np.concatenate([f._state_oxy for f in self._flamelets]))
+        fuel = np.ascontiguousarray(np.concatenate([f._state_fuel for f in self._flamelets]))
+        h = ctypes.c_void_p()
+        check(self._lib.gf_flamelet_batch_create(mech.griffon.handle, len(self._flamelets), iptr(self._nzi),
+                                                 float(f0._pressure), dptr(dz), dptr(chi), dptr(oxy), dptr(fuel),
+                                                 int(f0._include_enthalpy_flux), int(f0._include_variable_cp),
+                                                 ctypes.byref(h)))
+        self._h = h
+        self.set_state(np.concatenate([f._initial_state for f in self._flamelets]))
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            self._lib.gf_flamelet_batch_destroy(h)
+            self._h = None
+
+    def __len__(self):
+        return len(self._flamelets)
+
+    @property
+    def n_unknowns(self):
+        return int(self._offsets[-1])
+
+    def set_state(self, interior):
+        from ._native import check, dptr
+        x = np.ascontiguousarray(interior, dtype=np.float64)
+        if x.size != self.n_unknowns:
+            raise ValueError('state size %d, expected %d' % (x.size, self.n_unknowns))
+        check(self._lib.gf_flamelet_batch_set_state(self._h, dptr(x)))
+
+    def get_state(self):
+        from ._native import check, dptr
+        x = np.zeros(self.n_unknowns)
+        check(self._lib.gf_flamelet_batch_get_state(self._h, dptr(x)))
+        return x
+
+    def split(self, x):
+        return [x[self._offsets[k]:self._offsets[k + 1]].copy() for k in range(len(self._flamelets))]
+
+    def rhs(self):
+        """Flamelet RHS of every flamelet at the current state, concatenated."""
+        from ._native import check, dptr
+        out = np.zeros(self.n_unknowns)
+        check(self._lib.gf_flamelet_batch_rhs(self._h, dptr(out)))
+        return out
+
+    def compute_steady_state(self, tolerance=1.e-6, lu_precision=64, **psitc_args):
+        """Pseudo-transient continuation of every flamelet from its current state (psitc_args: the keyword
+        arguments of Flamelet.steady_solve_psitc). Returns (states, converged, iterations); raises if any
+        flamelet fails, unless psitc_args['raise_on_failure'] is False."""
+        from ._native import check, iptr, dptr, PsitcOptions
+        import ctypes
+        raise_on_failure = psitc_args.pop('raise_on_failure', True)
+        o = PsitcOptions()
+        self._lib.gf_psitc_options_default(ctypes.byref(o))
+        o.tolerance = tolerance
+        o.lu_precision = int(lu_precision)
+        names = {'max_iterations': 'max_iterations', 'ds_init': 'ds_init', 'ds_init_decrease': 'ds_init_decrease',
+                 'adaptive_restart': 'adaptive_restart', 'max_recursion_depth': 'max_recursion_depth',
+                 'diffusion_factor': 'diffusion_factor', 'global_ds': 'global_ds', 'ds_safety': 'ds_safety',
+                 'ds_ramp': 'ds_ramp', 'ds_max': 'ds_max', 'max_factor_line_search': 'max_factor_line_search',
+                 'max_allowed_residual': 'max_allowed_residual',
+                 'min_allowable_state_var': 'min_allowable_state_var'}
+        for k, v in psitc_args.items():
+            if k in ('verbose', 'log_rate', 'norm_order'):
+                continue
+            if k not in names:
+                raise TypeError('unknown psitc argument ' + k)
+            setattr(o, names[k], type(getattr(o, names[k]))(v))
+        B = len(self._flamelets)
+        it = np.zeros(B, dtype=np.int32)
+        conv = np.zeros(B, dtype=np.int32)
+        res = np.zeros(B)
+        cnt = (ctypes.c_longlong * 4)()
+        check(self._lib.gf_flamelet_batch_steady_psitc(self._h, ctypes.byref(o), iptr(it), iptr(conv), dptr(res),
+                                                       cnt))
+        self.counters = {'iterations': cnt[0], 'jacobian_refreshes': cnt[1], 'rhs_evaluations': cnt[2],
+                         'synchronizing_steps': cnt[3]}
+        states = self.split(self.get_state())
+        for f, s, c, n in zip(self._flamelets, states, conv, it):
+            if c:
+                f._current_state = s.copy()
+                f._iteration_count = int(n)
+        self.residuals = res
+        if raise_on_failure and not np.all(conv):
+            raise RuntimeError('steady flamelet solve did not converge for %d of %d flamelets'
+                               % (int(B - conv.sum()), B))
+        return states, conv.astype(bool), it

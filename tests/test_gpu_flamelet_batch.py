@@ -1,0 +1,70 @@
+"""The device-resident batched steady flamelet solver (gf_flamelet_batch_*, spitfire_amd.flamelet.FlameletBatch)
+against the oracle: the batched flamelet RHS against the oracle's flamelet_rhs for each flamelet, and converged
+steady states against the oracle's own pseudo-transient continuation (the reference algorithm with LAPACK
+eigenvalues and LU, oracle/flamelet_oracle.py), both solved to a scaled residual of 1e-10.
+
+Bound on converged states: max |x_gpu - x_oracle| / scale <= 1e-7, scale 1e3 for temperature and 1 for mass
+fractions; two solves of the same steady problem to residual 1e-10 differ by at most about the residual times
+the inverse Jacobian norm."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _streams(m):
+    return m.stream(stp_air=True), m.stream('TPY', (300., 101325., 'H2:1 N2:1'))
+
+
+def _flamelets(m, chis, npts, cls=None):
+    from spitfire_amd.flamelet import Flamelet, FlameletSpec
+    cls = cls or Flamelet
+    air, fuel = _streams(m)
+    return [cls(FlameletSpec(mech_spec=m, oxy_stream=air, fuel_stream=fuel, grid_points=n,
+                             stoich_dissipation_rate=c)) for c, n in zip(chis, npts)]
+
+
+@pytest.mark.parametrize('mech', ['mech_c1_data', 'mech_53_data'])
+def test_batch_rhs_matches_oracle(mech, request, oracle_lib):
+    from spitfire_amd.flamelet import FlameletBatch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from oracle.flamelet_oracle import OracleFlamelet
+    m = ChemicalMechanismSpec(request.getfixturevalue(mech))
+    chis, npts = [0.5, 20., 300.], [9, 34, 17]
+    say('batch rhs', mech, npts)
+    fb = FlameletBatch(_flamelets(m, chis, npts))
+    got = fb.split(fb.rhs())
+    for f, g in zip(_flamelets(m, chis, npts, OracleFlamelet), got):
+        ref = f.rhs(0., f.initial_interior_state)
+        ns = m.n_species
+        e = np.max(np.abs(g - ref).reshape(-1, ns).max(axis=1) /
+                   np.maximum(np.abs(ref).reshape(-1, ns).max(axis=1), 1e-300))
+        say('  nzi %d row err %.3e' % (f.n_interior_points, e))
+        assert e <= 1e-12
+
+
+@pytest.mark.parametrize('mech, chis', [('mech_c1_data', [1., 10., 100., 1000.]), ('mech_53_data', [1., 10.])])
+def test_batch_steady_states_match_oracle(mech, chis, request, oracle_lib):
+    from spitfire_amd.flamelet import FlameletBatch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from oracle.flamelet_oracle import OracleFlamelet
+    m = ChemicalMechanismSpec(request.getfixturevalue(mech))
+    ns = m.n_species
+    npts = [34] * len(chis)
+    say('batch steady', mech, chis)
+    fb = FlameletBatch(_flamelets(m, chis, npts))
+    states, conv, its = fb.compute_steady_state(tolerance=1e-10)
+    say('  gpu iterations', its.tolist(), 'counters', fb.counters, 'residuals', fb.residuals.tolist())
+    assert conv.all()
+    for f, g in zip(_flamelets(m, chis, npts, OracleFlamelet), states):
+        ref, it, ok, _ = f.steady_solve_psitc(tolerance=1e-10)
+        assert ok
+        scale = np.tile(np.r_[1e3, np.ones(ns - 1)], f.n_interior_points)
+        e = np.max(np.abs(g - ref) / scale)
+        say('  chi_st %.3g Tmax gpu %.2f oracle %.2f max scaled diff %.3e (oracle %d iterations)'
+            % (f.dissipation_rate.max(), g[::ns].max(), ref[::ns].max(), e, it))
+        assert e <= 1e-7

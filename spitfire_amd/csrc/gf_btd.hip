@@ -1,4 +1,6 @@
-// Batched block-tridiagonal factor/solve with diagonal off-diagonal blocks (griffon's BTDDOD format), FP64.
+// Batched block-tridiagonal factor/solve with diagonal off-diagonal blocks (griffon's BTDDOD format), with
+// FP64 factors (in place, as the reference) or FP32 factors in separate arrays (the mixed-precision path of
+// the flamelet solvers: FP32 block LU, FP64 right-hand sides, solutions and residuals).
 //
 // Algorithm restated from btddod_matrix_kernels.cpp:19-119 (block Thomas):
 //   D_0 = LU(D_0);  for i >= 1:  L_i = diag(sub_{i-1}) D_{i-1}^{-1},  D_i -= L_i diag(sup_{i-1}),  D_i = LU(D_i)
@@ -20,7 +22,8 @@ namespace {
 constexpr int BT_THREADS = 256;
 
 // LU with partial pivoting of a bs x bs column-major block in LDS (LAPACK dgetf2 order of operations).
-__device__ void lds_lu(double *A, int bs, int *piv_lds, int *gpiv) {
+template <typename T>
+__device__ void lds_lu(T *A, int bs, int *piv_lds, int *gpiv) {
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   __shared__ int s_p;
@@ -28,7 +31,7 @@ __device__ void lds_lu(double *A, int bs, int *piv_lds, int *gpiv) {
     if (wave == 0) {
       double v = -1.0;
       int idx = bs;
-      if (lane >= j && lane < bs) { v = fabs(A[j * bs + lane]); idx = lane; }
+      if (lane >= j && lane < bs) { v = fabs((double)A[j * bs + lane]); idx = lane; }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
         const double ov = __shfl_xor(v, o, 64);
@@ -46,14 +49,14 @@ __device__ void lds_lu(double *A, int bs, int *piv_lds, int *gpiv) {
     const int p = s_p;
     if (p != j)
       for (int c = t; c < bs; c += BT_THREADS) {
-        const double a0 = A[c * bs + j];
+        const T a0 = A[c * bs + j];
         A[c * bs + j] = A[c * bs + p];
         A[c * bs + p] = a0;
       }
     __syncthreads();
-    const double d = A[j * bs + j];
-    if (d != 0.) {
-      const double id = 1. / d;
+    const T d = A[j * bs + j];
+    if (d != T(0)) {
+      const T id = T(1) / d;
       for (int k = j + 1 + t; k < bs; k += BT_THREADS) A[j * bs + k] *= id;
     }
     __syncthreads();
@@ -67,13 +70,14 @@ __device__ void lds_lu(double *A, int bs, int *piv_lds, int *gpiv) {
 }
 
 // X <- A^{-1} X for bs right-hand sides, A an LU factorization with pivots in LDS (LAPACK dgetrs, 'N').
-__device__ void lds_lu_solve_multi(const double *A, const int *piv, int bs, double *X) {
+template <typename T>
+__device__ void lds_lu_solve_multi(const T *A, const int *piv, int bs, T *X) {
   const int t = threadIdx.x;
   for (int j = 0; j < bs; ++j) {  // row interchanges, applied in order
     const int p = piv[j] - 1;
     if (p != j)
       for (int c = t; c < bs; c += BT_THREADS) {
-        const double a0 = X[c * bs + j];
+        const T a0 = X[c * bs + j];
         X[c * bs + j] = X[c * bs + p];
         X[c * bs + p] = a0;
       }
@@ -98,56 +102,65 @@ __device__ void lds_lu_solve_multi(const double *A, const int *piv, int bs, doub
   }
 }
 
+// Factors of system sys: D blocks to dout + doff[sys] (for FP64 the values array itself, in place), L blocks to
+// lout + loff[sys]; the input blocks and off-diagonals are the FP64 values array.
+template <typename T>
 __global__ void __launch_bounds__(BT_THREADS) btd_factor_kernel(const int *nblocks, int bs, double *values,
-                                                               const long *voff, double *lval, const long *loff,
-                                                               int *piv, const long *poff) {
-  extern __shared__ double sm[];
+                                                               const long *voff, T *dout, const long *doff, T *lout,
+                                                               const long *loff, int *piv, const long *poff) {
+  extern __shared__ double sm_raw[];
+  T *sm = reinterpret_cast<T *>(sm_raw);
   const int sys = blockIdx.x;
   const int nb = nblocks[sys];
   const int nb2 = bs * bs;
-  double *V = values + voff[sys];
-  double *L = lval + loff[sys];
+  const double *V = values + voff[sys];
+  T *Dg = dout + doff[sys];
+  T *L = lout + loff[sys];
   int *P = piv + poff[sys];
   const double *sub = V + (long)nb * nb2;
   const double *sup = sub + (long)(nb - 1) * bs;
-  double *D = sm, *Lb = sm + nb2;                // factored diagonal block, L block
+  T *D = sm, *Lb = sm + nb2;                // factored diagonal block, L block
   int *pv = reinterpret_cast<int *>(sm + 2 * nb2);
   const int t = threadIdx.x;
 
-  for (int e = t; e < nb2; e += BT_THREADS) D[e] = V[e];
+  for (int e = t; e < nb2; e += BT_THREADS) D[e] = (T)V[e];
   __syncthreads();
   lds_lu(D, bs, pv, P);
-  for (int e = t; e < nb2; e += BT_THREADS) V[e] = D[e];
+  for (int e = t; e < nb2; e += BT_THREADS) Dg[e] = D[e];
   for (int i = 1; i < nb; ++i) {
     const long o1 = (long)i * nb2;
-    for (int e = t; e < nb2; e += BT_THREADS) Lb[e] = ((e % bs) == (e / bs)) ? 1. : 0.;
+    for (int e = t; e < nb2; e += BT_THREADS) Lb[e] = ((e % bs) == (e / bs)) ? T(1) : T(0);
     __syncthreads();
     lds_lu_solve_multi(D, pv, bs, Lb);  // Lb = D_{i-1}^{-1}
     const int b1 = (i - 1) * bs;
     for (int e = t; e < nb2; e += BT_THREADS) {
       const int j = e / bs, k = e % bs;
-      const double l = Lb[e] * sub[b1 + k];
+      const T l = Lb[e] * (T)sub[b1 + k];
       L[o1 + e] = l;
-      D[e] = V[o1 + e] + (-sup[b1 + j]) * l;
+      D[e] = (T)(V[o1 + e] + (-sup[b1 + j]) * (double)l);
     }
     __syncthreads();
     lds_lu(D, bs, pv, P + i * bs);
-    for (int e = t; e < nb2; e += BT_THREADS) V[o1 + e] = D[e];
+    for (int e = t; e < nb2; e += BT_THREADS) Dg[o1 + e] = D[e];
   }
 }
 
 // one wave per system
+// (FP64 arithmetic; FP32 factors are widened on load)
+template <typename T>
 __global__ void __launch_bounds__(64) btd_solve_kernel(int nsys, const int *nblocks, int bs, const double *values,
-                                                       const long *voff, const double *lval, const long *loff,
-                                                       const int *piv, const long *poff, const double *rhs,
-                                                       double *x, const long *xoff) {
+                                                       const long *voff, const T *dfac, const long *doff,
+                                                       const T *lval, const long *loff, const int *piv,
+                                                       const long *poff, const double *rhs, double *x,
+                                                       const long *xoff) {
   const int sys = blockIdx.x;
   if (sys >= nsys) return;
   const int lane = threadIdx.x;
   const int nb = nblocks[sys];
   const int nb2 = bs * bs;
   const double *V = values + voff[sys];
-  const double *L = lval + loff[sys];
+  const T *DF = dfac + doff[sys];
+  const T *L = lval + loff[sys];
   const int *P = piv + poff[sys];
   const double *b = rhs + xoff[sys];
   double *X = x + xoff[sys];
@@ -158,11 +171,11 @@ __global__ void __launch_bounds__(64) btd_solve_kernel(int nsys, const int *nblo
   double y = act ? b[lane] : 0.;
   if (act) X[lane] = y;
   for (int i = 1; i < nb; ++i) {
-    const double *Li = L + (long)i * nb2;
+    const T *Li = L + (long)i * nb2;
     double yi = act ? b[(long)i * bs + lane] : 0.;
     for (int c = 0; c < bs; ++c) {
       const double yc = __shfl(y, c, 64);
-      if (act) yi = yi + Li[c * bs + lane] * (-1. * yc);
+      if (act) yi = yi + (double)Li[c * bs + lane] * (-1. * yc);
     }
     y = yi;
     if (act) X[(long)i * bs + lane] = y;
@@ -170,7 +183,7 @@ __global__ void __launch_bounds__(64) btd_solve_kernel(int nsys, const int *nblo
   // backward
   double xn = 0.;
   for (int i = nb - 1; i >= 0; --i) {
-    const double *D = V + (long)i * nb2;
+    const T *D = DF + (long)i * nb2;
     const int *Pi = P + i * bs;
     double r = act ? X[(long)i * bs + lane] : 0.;
     if (i < nb - 1 && act) r = r - sup[(long)i * bs + lane] * xn;
@@ -184,12 +197,12 @@ __global__ void __launch_bounds__(64) btd_solve_kernel(int nsys, const int *nblo
     }
     for (int j = 0; j < bs; ++j) {  // unit lower
       const double rj = __shfl(r, j, 64);
-      if (act && lane > j) r -= D[j * bs + lane] * rj;
+      if (act && lane > j) r -= (double)D[j * bs + lane] * rj;
     }
     for (int j = bs - 1; j >= 0; --j) {  // upper
-      if (lane == j) r /= D[j * bs + j];
+      if (lane == j) r /= (double)D[j * bs + j];
       const double rj = __shfl(r, j, 64);
-      if (act && lane < j) r -= D[j * bs + lane] * rj;
+      if (act && lane < j) r -= (double)D[j * bs + lane] * rj;
     }
     xn = r;
     if (act) X[(long)i * bs + lane] = r;
@@ -200,28 +213,49 @@ __global__ void __launch_bounds__(64) btd_solve_kernel(int nsys, const int *nblo
 
 int launch_btd_factor(int nsys, const int *nblocks, int bs, double *values, const long *voff, double *l_values,
                       const long *loff, int *pivots, const long *poff, hipStream_t st) {
-  if (nsys <= 0) return GF_OK;
-  if (bs < 1 || bs > 64) return fail(GF_ERR_UNSUPPORTED, "block size must be in 1..64");
-  const size_t lds = sizeof(double) * 2 * bs * bs + sizeof(int) * 64;
-  if (lds > 65536)
-    if (hipFuncSetAttribute((const void *)btd_factor_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-        hipSuccess)
-      return fail(GF_ERR_UNSUPPORTED, "block too large for LDS");
-  hipLaunchKernelGGL(btd_factor_kernel, dim3(nsys), dim3(BT_THREADS), lds, st, nblocks, bs, values, voff, l_values,
-                     loff, pivots, poff);
-  if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, "btd factor launch failed");
-  return GF_OK;
+  return launch_btd_factor_t<double>(nsys, nblocks, bs, values, voff, values, voff, l_values, loff, pivots, poff, st);
 }
 
 int launch_btd_solve(int nsys, const int *nblocks, int bs, const double *values, const long *voff,
                      const double *l_values, const long *loff, const int *pivots, const long *poff, const double *rhs,
                      double *x, const long *xoff, hipStream_t st) {
+  return launch_btd_solve_t<double>(nsys, nblocks, bs, values, voff, values, voff, l_values, loff, pivots, poff, rhs,
+                                    x, xoff, st);
+}
+
+template <typename T>
+int launch_btd_factor_t(int nsys, const int *nblocks, int bs, double *values, const long *voff, T *dfac,
+                        const long *doff, T *l_values, const long *loff, int *pivots, const long *poff,
+                        hipStream_t st) {
   if (nsys <= 0) return GF_OK;
   if (bs < 1 || bs > 64) return fail(GF_ERR_UNSUPPORTED, "block size must be in 1..64");
-  hipLaunchKernelGGL(btd_solve_kernel, dim3(nsys), dim3(64), 0, st, nsys, nblocks, bs, values, voff, l_values, loff,
-                     pivots, poff, rhs, x, xoff);
+  const size_t lds = sizeof(T) * 2 * bs * bs + sizeof(int) * 64 + 16;
+  auto kern = btd_factor_kernel<T>;
+  if (lds > 65536)
+    if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+      return fail(GF_ERR_UNSUPPORTED, "block too large for LDS");
+  hipLaunchKernelGGL(kern, dim3(nsys), dim3(BT_THREADS), lds, st, nblocks, bs, values, voff, dfac, doff, l_values,
+                     loff, pivots, poff);
+  if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, "btd factor launch failed");
+  return GF_OK;
+}
+
+template <typename T>
+int launch_btd_solve_t(int nsys, const int *nblocks, int bs, const double *values, const long *voff, const T *dfac,
+                       const long *doff, const T *l_values, const long *loff, const int *pivots, const long *poff,
+                       const double *rhs, double *x, const long *xoff, hipStream_t st) {
+  if (nsys <= 0) return GF_OK;
+  if (bs < 1 || bs > 64) return fail(GF_ERR_UNSUPPORTED, "block size must be in 1..64");
+  hipLaunchKernelGGL(btd_solve_kernel<T>, dim3(nsys), dim3(64), 0, st, nsys, nblocks, bs, values, voff, dfac, doff,
+                     l_values, loff, pivots, poff, rhs, x, xoff);
   if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, "btd solve launch failed");
   return GF_OK;
 }
+
+template int launch_btd_factor_t<float>(int, const int *, int, double *, const long *, float *, const long *, float *,
+                                        const long *, int *, const long *, hipStream_t);
+template int launch_btd_solve_t<float>(int, const int *, int, const double *, const long *, const float *,
+                                       const long *, const float *, const long *, const int *, const long *,
+                                       const double *, double *, const long *, hipStream_t);
 
 }  // namespace gfa

@@ -56,6 +56,7 @@ struct gf_flamelet_batch {
          *eig = nullptr, *J = nullptr, *L = nullptr, *chem_rhs = nullptr, *aux = nullptr, *cpi = nullptr,
          *cmaj = nullptr, *csub = nullptr, *csup = nullptr, *mco = nullptr, *nco = nullptr, *chi = nullptr,
          *dz = nullptr, *dp = nullptr, *fval = nullptr, *fres = nullptr, *fmin = nullptr, *fdiff = nullptr;
+  float *D32 = nullptr, *L32 = nullptr;  // FP32 block-LU factors (mixed-precision path)
   int *piv = nullptr, *dnzi = nullptr, *flag = nullptr, *gmin = nullptr, *sel = nullptr, *sys_nb = nullptr;
   long *dpt0 = nullptr, *ddof0 = nullptr, *dipt0 = nullptr, *dchi0 = nullptr, *dvoff = nullptr, *ddz0 = nullptr,
        *js_off = nullptr, *eb = nullptr, *eo = nullptr, *sys_v = nullptr, *sys_l = nullptr, *sys_p = nullptr,
@@ -93,6 +94,8 @@ size_t carve_all(gf_flamelet_batch *b, char *base) {
   b->eig = c.take<double>(b->NV);
   b->J = c.take<double>(b->NJ);
   b->L = c.take<double>(b->NL);
+  b->D32 = c.take<float>(b->NL);
+  b->L32 = c.take<float>(b->NL);
   b->chem_rhs = c.take<double>(b->NP * ns);
   b->aux = c.take<double>(b->NP * 4);
   b->cpi = c.take<double>(b->NP * ns);
@@ -255,12 +258,18 @@ struct Psitc {
     GF_TRY(launch_fb_ds(b->view, b->flag, b->gmin, b->eig, b->fdiff, o.ds_safety, o.ds_ramp, o.ds_max, b->ds, st));
     GF_TRY(launch_fb_jac_psitc(b->m, b->view, b->X, b->flag, b->ds, b->J, st));
     GF_TRY(systems(fl));
+    if (o.lu_precision == 32)
+      return launch_btd_factor_t<float>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->D32, b->sys_l, b->L32,
+                                        b->sys_l, b->piv, b->sys_p, st);
     return launch_btd_factor((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->L, b->sys_l, b->piv, b->sys_p, st);
   }
 
   int solve(const std::vector<int> &fl) {
     if (fl.empty()) return GF_OK;
     GF_TRY(systems(fl));
+    if (o.lu_precision == 32)
+      return launch_btd_solve_t<float>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->D32, b->sys_l, b->L32,
+                                       b->sys_l, b->piv, b->sys_p, b->R, b->dX, b->sys_x, st);
     return launch_btd_solve((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->L, b->sys_l, b->piv, b->sys_p, b->R,
                             b->dX, b->sys_x, st);
   }
@@ -587,7 +596,7 @@ int gf_flamelet_batch_steady_psitc(gf_flamelet_batch *b, const gf_psitc_options 
   gf_psitc_options o;
   gf_psitc_options_default(&o);
   if (opts) o = *opts;
-  if (o.lu_precision != 64) return fail(GF_ERR_UNSUPPORTED, "only the FP64 block LU is available");
+  if (o.lu_precision != 64 && o.lu_precision != 32) return fail(GF_ERR_ARG, "lu_precision must be 64 or 32");
   std::vector<double> d(b->B);
   for (int f = 0; f < b->B; ++f) d[f] = o.diffusion_factor * b->diffmax[f];
   GF_TRY(up(b->fdiff, d, b->m->stream));

@@ -210,4 +210,13 @@ int launch_btd_factor(int nsys, const int *nblocks, int bs, double *values, cons
 int launch_btd_solve(int nsys, const int *nblocks, int bs, const double *values, const long *voff,
                      const double *l_values, const long *loff, const int *pivots, const long *poff,
                      const double *rhs, double *x, const long *xoff, hipStream_t st);
+// the same with the D factors in dfac + doff[s] (FP64 in place: dfac = values, doff = voff) and factor type T
+template <typename T>
+int launch_btd_factor_t(int nsys, const int *nblocks, int bs, double *values, const long *voff, T *dfac,
+                        const long *doff, T *l_values, const long *loff, int *pivots, const long *poff,
+                        hipStream_t st);
+template <typename T>
+int launch_btd_solve_t(int nsys, const int *nblocks, int bs, const double *values, const long *voff, const T *dfac,
+                       const long *doff, const T *l_values, const long *loff, const int *pivots, const long *poff,
+                       const double *rhs, double *x, const long *xoff, hipStream_t st);
 }  // namespace gfa

@@ -68,3 +68,25 @@ def test_batch_steady_states_match_oracle(mech, chis, request, oracle_lib):
         say('  chi_st %.3g Tmax gpu %.2f oracle %.2f max scaled diff %.3e (oracle %d iterations)'
             % (f.dissipation_rate.max(), g[::ns].max(), ref[::ns].max(), e, it))
         assert e <= 1e-7
+
+
+@pytest.mark.parametrize('mech, chis', [('mech_c1_data', [1., 10., 100., 1000.]), ('mech_53_data', [1., 10.])])
+def test_mixed_precision_lu_converges_to_fp64_solution(mech, chis, request):
+    """FP32 block-LU factors with FP64 residuals reach the FP64-LU steady state (to 1e-8 scaled, both solved to
+    a scaled residual of 1e-10): the factors only precondition the FP64 residual."""
+    from spitfire_amd.flamelet import FlameletBatch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    m = ChemicalMechanismSpec(request.getfixturevalue(mech))
+    ns = m.n_species
+    npts = [34] * len(chis)
+    say('mixed precision', mech, chis)
+    s64, c64, i64 = FlameletBatch(_flamelets(m, chis, npts)).compute_steady_state(tolerance=1e-10, lu_precision=64)
+    fb = FlameletBatch(_flamelets(m, chis, npts))
+    s32, c32, i32 = fb.compute_steady_state(tolerance=1e-10, lu_precision=32)
+    say('  iterations fp64', i64.tolist(), 'fp32', i32.tolist(), 'residuals', fb.residuals.tolist())
+    assert c64.all() and c32.all()
+    for a, b_ in zip(s32, s64):
+        scale = np.tile(np.r_[1e3, np.ones(ns - 1)], a.size // ns)
+        e = np.max(np.abs(a - b_) / scale)
+        say('  max scaled diff %.3e' % e)
+        assert e <= 1e-8

@@ -1,11 +1,12 @@
 """The device-resident batched steady flamelet solver (gf_flamelet_batch_*, spitfire_amd.flamelet.FlameletBatch)
 against the oracle: the batched flamelet RHS against the oracle's flamelet_rhs for each flamelet, and converged
 steady states against the oracle's own pseudo-transient continuation (the reference algorithm with LAPACK
-eigenvalues and LU, oracle/flamelet_oracle.py), both solved to a scaled residual of 1e-10.
+eigenvalues and LU, oracle/flamelet_oracle.py), both solved to a scaled residual of 1e-8 (the FP64 rounding
+floor of the scaled flamelet residual is near 5e-9 on 128-point clustered grids, so 1e-10 is not reachable in
+general).
 
-Bound on converged states: max |x_gpu - x_oracle| / scale <= 1e-7, scale 1e3 for temperature and 1 for mass
-fractions; two solves of the same steady problem to residual 1e-10 differ by at most about the residual times
-the inverse Jacobian norm."""
+Bound on converged states: max |x_gpu - x_oracle| / scale <= 1e-8, scale 1e3 for temperature and 1 for mass
+fractions (the oracle's own solutions at residual 1e-8 and 1e-9 differ by under 1e-10 on these cases)."""
 import numpy as np
 import pytest
 
@@ -57,32 +58,32 @@ def test_batch_steady_states_match_oracle(mech, chis, request, oracle_lib):
     npts = [34] * len(chis)
     say('batch steady', mech, chis)
     fb = FlameletBatch(_flamelets(m, chis, npts))
-    states, conv, its = fb.compute_steady_state(tolerance=1e-10)
+    states, conv, its = fb.compute_steady_state(tolerance=1e-8)
     say('  gpu iterations', its.tolist(), 'counters', fb.counters, 'residuals', fb.residuals.tolist())
     assert conv.all()
     for f, g in zip(_flamelets(m, chis, npts, OracleFlamelet), states):
-        ref, it, ok, _ = f.steady_solve_psitc(tolerance=1e-10)
+        ref, it, ok, _ = f.steady_solve_psitc(tolerance=1e-8)
         assert ok
         scale = np.tile(np.r_[1e3, np.ones(ns - 1)], f.n_interior_points)
         e = np.max(np.abs(g - ref) / scale)
         say('  chi_st %.3g Tmax gpu %.2f oracle %.2f max scaled diff %.3e (oracle %d iterations)'
             % (f.dissipation_rate.max(), g[::ns].max(), ref[::ns].max(), e, it))
-        assert e <= 1e-7
+        assert e <= 1e-8
 
 
 @pytest.mark.parametrize('mech, chis', [('mech_c1_data', [1., 10., 100., 1000.]), ('mech_53_data', [1., 10.])])
 def test_mixed_precision_lu_converges_to_fp64_solution(mech, chis, request):
     """FP32 block-LU factors with FP64 residuals reach the FP64-LU steady state (to 1e-8 scaled, both solved to
-    a scaled residual of 1e-10): the factors only precondition the FP64 residual."""
+    a scaled residual of 1e-8): the factors only precondition the FP64 residual."""
     from spitfire_amd.flamelet import FlameletBatch
     from spitfire_amd.mechanism import ChemicalMechanismSpec
     m = ChemicalMechanismSpec(request.getfixturevalue(mech))
     ns = m.n_species
     npts = [34] * len(chis)
     say('mixed precision', mech, chis)
-    s64, c64, i64 = FlameletBatch(_flamelets(m, chis, npts)).compute_steady_state(tolerance=1e-10, lu_precision=64)
+    s64, c64, i64 = FlameletBatch(_flamelets(m, chis, npts)).compute_steady_state(tolerance=1e-8, lu_precision=64)
     fb = FlameletBatch(_flamelets(m, chis, npts))
-    s32, c32, i32 = fb.compute_steady_state(tolerance=1e-10, lu_precision=32)
+    s32, c32, i32 = fb.compute_steady_state(tolerance=1e-8, lu_precision=32)
     say('  iterations fp64', i64.tolist(), 'fp32', i32.tolist(), 'residuals', fb.residuals.tolist())
     assert c64.all() and c32.all()
     for a, b_ in zip(s32, s64):

@@ -158,9 +158,11 @@ class Flamelet:
             for i in range(neq):
                 st[:, i] = self._state_oxy[i] + (self._state_fuel[i] - self._state_oxy[i]) * zi
         elif ic == 'unreacted':
-            for i in range(nzi):
-                m = mech.mix_streams([(self._oxy_stream, 1 - zi[i]), (self._fuel_stream, zi[i])], 'mass', 'HP')
-                st[i, :] = m.state
+            Yo, Yf = self._oxy_stream.Y, self._fuel_stream.Y
+            Y = Yo[None, :] + zi[:, None] * (Yf - Yo)[None, :]
+            h = (1. - zi) * self._oxy_stream.H + zi * self._fuel_stream.H
+            st[:, 0] = mech.temperatures_from_enthalpies(h, Y, T0=self._oxy_stream.T)
+            st[:, 1:] = Y[:, :-1]
         elif ic == 'Burke-Schumann':
             zst = mech.stoich_mixture_fraction(self._fuel_stream, self._oxy_stream)
             stmix = mech.mix_streams([(self._oxy_stream, 1 - zst), (self._fuel_stream, zst)], 'mass', 'HP')
@@ -171,11 +173,13 @@ class Flamelet:
                 comp += ' CO2: %.17g' % stat.get('C', 0.)
             sw = mech.stream('HPX', (stmix.H, self._pressure, comp))
             Yst, Yo, Yf = sw.Y, self._oxy_stream.Y, self._fuel_stream.Y
-            for i in range(nzi):
-                z = zi[i]
-                h = mech.mix_streams([(self._oxy_stream, 1 - z), (self._fuel_stream, z)], 'mass', 'HP').H
-                Y = Yo + z / zst * (Yst - Yo) if z <= zst else Yst + (z - zst) / (1. - zst) * (Yf - Yst)
-                st[i, :] = mech.stream('HPY', (h, self._pressure, Y)).state
+            lean = zi <= zst
+            Y = np.where(lean[:, None], Yo[None, :] + (zi / zst)[:, None] * (Yst - Yo)[None, :],
+                         Yst[None, :] + ((zi - zst) / (1. - zst))[:, None] * (Yf - Yst)[None, :])
+            # mass-basis mixing at constant HP (mix_streams): the enthalpy is linear in Z
+            h = (1. - zi) * self._oxy_stream.H + zi * self._fuel_stream.H
+            st[:, 0] = mech.temperatures_from_enthalpies(h, Y)
+            st[:, 1:] = Y[:, :-1]
         else:
             raise ValueError('unsupported initial condition ' + str(ic))
         return st.ravel().copy()

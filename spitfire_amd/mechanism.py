@@ -220,6 +220,65 @@ class ChemicalMechanismSpec:
             cpv[i] = R * (a[0] + Te * (a[1] + Te * (a[2] + Te * (a[3] + Te * a[4])))) / self._mw[i]
         return cpv
 
+    def _thermo_tables(self):
+        """NASA7 / constant-cp coefficients as arrays, for the vectorized thermo below"""
+        if getattr(self, '_tt', None) is None:
+            ns = self.n_species
+            lo, hi = np.zeros((ns, 7)), np.zeros((ns, 7))
+            tmid, tmin, tmax = np.zeros(ns), np.zeros(ns), np.zeros(ns)
+            const = np.zeros(ns, dtype=bool)
+            c0 = np.zeros((ns, 3))  # T0, h0, cp of constant-cp species
+            for i, n in enumerate(self._species_names):
+                cp = self._mech_data['species'][n]['heat-capacity']
+                if cp['type'] == 'constant':
+                    const[i] = True
+                    c0[i] = cp['T0'], cp['h0'], cp['cp']
+                    continue
+                lo[i], hi[i] = cp['low-coeffs'], cp['high-coeffs']
+                tmid[i], tmin[i], tmax[i] = cp['Tmid'], cp['Tmin'], cp['Tmax']
+            self._tt = (lo, hi, tmid, tmin, tmax, const, c0)
+        return self._tt
+
+    def species_enthalpies_array(self, T):
+        """species enthalpies per unit mass [len(T), ns] for an array of temperatures (species_enthalpies
+        applied row by row)"""
+        lo, hi, tmid, tmin, tmax, const, c0 = self._thermo_tables()
+        T = np.asarray(T, dtype=np.float64)[:, None]
+        Te = np.minimum(np.maximum(T, tmin[None, :]), tmax[None, :])
+        a = np.where((Te <= tmid[None, :])[:, :, None], lo[None, :, :], hi[None, :, :])
+        hRT = (a[..., 0] * Te + a[..., 1] * Te ** 2 / 2 + a[..., 2] * Te ** 3 / 3 + a[..., 3] * Te ** 4 / 4 +
+               a[..., 4] * Te ** 5 / 5 + a[..., 5])
+        cpe = a[..., 0] + Te * (a[..., 1] + Te * (a[..., 2] + Te * (a[..., 3] + Te * a[..., 4])))
+        h = self.gas_constant * (hRT + cpe * (T - Te)) / self._mw[None, :]
+        if const.any():
+            h[:, const] = (c0[const, 1] + c0[const, 2] * (T - c0[const, 0])) / self._mw[const]
+        return h
+
+    def species_cp_array(self, T):
+        lo, hi, tmid, tmin, tmax, const, c0 = self._thermo_tables()
+        T = np.asarray(T, dtype=np.float64)[:, None]
+        Te = np.minimum(np.maximum(T, tmin[None, :]), tmax[None, :])
+        a = np.where((Te <= tmid[None, :])[:, :, None], lo[None, :, :], hi[None, :, :])
+        cp = self.gas_constant * (a[..., 0] + Te * (a[..., 1] + Te * (a[..., 2] + Te * (a[..., 3] + Te * a[..., 4]))))
+        cp = cp / self._mw[None, :]
+        if const.any():
+            cp[:, const] = c0[const, 2] / self._mw[const]
+        return cp
+
+    def temperatures_from_enthalpies(self, h, Y, T0=1000., tol=1e-12):
+        """temperature_from_enthalpy for many mixtures at once (h [n], Y [n, ns])"""
+        h = np.asarray(h, dtype=np.float64)
+        T = np.full(h.shape, float(T0)) if np.ndim(T0) == 0 else np.array(T0, dtype=np.float64)
+        done = np.zeros(h.shape, dtype=bool)
+        for _ in range(100):
+            f = np.sum(self.species_enthalpies_array(T) * Y, axis=1) - h
+            dT = f / np.sum(self.species_cp_array(T) * Y, axis=1)
+            T = np.where(done, T, T - dT)
+            done |= np.abs(dT) < tol * T
+            if done.all():
+                break
+        return T
+
     def temperature_from_enthalpy(self, h, Y, T0=1000., tol=1e-12):
         """T such that h_mix(T, Y) = h, by Newton iteration in FP64."""
         T = T0

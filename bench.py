@@ -1,4 +1,5 @@
-"""Benchmark: batched isobaric RHS + analytic Jacobian evaluations per second on one MI355X.
+"""Benchmark: batched isobaric RHS + analytic Jacobian evaluations per second on one MI355X, and steady flamelet
+solves per second with FP64 and FP32 block-LU factors.
 
 Workload (the headline RHS+Jac configuration): the seeded synthetic 53-species / 325-reaction H/O/N mechanism
 evaluated over 262,144 independent states (T uniform in 300-2500 K, p log-uniform in 0.1-10 atm, Y ~
@@ -11,7 +12,15 @@ evaluation = 8 * (ns + 1 + ns + ns^2) = 23,328 B: state and pressure in, RHS and
 reference's own griffon build (oracle/_ref) timed on this host's cores on a bounded sample as the CPU
 baseline.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+Flamelets (the "flamelets" object of the JSON line): the 53-species mechanism, H2:N2 1:1 (mass) fuel against
+air, both at 300 K and 1 atm, Burke-Schumann initial states, pseudo-transient continuation to a scaled residual
+of 1e-8 with the device-resident batched solver:
+  - one flamelet, 128-point grid clustered at Z_st, chi_st = 10 1/s;
+  - 512 flamelets, grid sizes uniform in 64..256 points, chi_st log-uniform in 1e-2..1e3 1/s (seed 0);
+each with FP64 and with FP32 block-LU factors (FP64 residuals), the agreement between the two, and the
+one-flamelet case solved by the oracle (the reference algorithm on the CPU) as the flamelet CPU baseline.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-flamelets]
 """
 import argparse
 import json
@@ -52,6 +61,76 @@ def cpu_baseline(data, x, p, sample, threads):
             'single_core_value': n1 / dt1}
 
 
+def flamelet_cases(spec, seed=0, n_library=512):
+    import numpy as np
+    from spitfire_amd.flamelet import Flamelet, FlameletSpec
+    air = spec.stream(stp_air=True)
+    fuel = spec.stream('TPY', (300., 101325., 'H2:1 N2:1'))
+
+    def make(chi, npts):
+        return Flamelet(FlameletSpec(mech_spec=spec, oxy_stream=air, fuel_stream=fuel, grid_points=int(npts),
+                                     stoich_dissipation_rate=float(chi)))
+
+    rng = np.random.default_rng(seed)
+    npts = rng.integers(64, 257, n_library)
+    chis = 10. ** rng.uniform(-2., 3., n_library)
+    return (lambda: [make(10., 128)]), (lambda: [make(c, n) for c, n in zip(chis, npts)]), (air, fuel)
+
+
+def scaled_diff(a, b, ns):
+    import numpy as np
+    scale = np.tile(np.r_[1e3, np.ones(ns - 1)], a.size // ns)
+    return float(np.max(np.abs(a - b) / scale))
+
+
+def run_flamelets(spec, tol=1e-8, cpu=True):
+    import numpy as np
+    import torch
+    from spitfire_amd.flamelet import FlameletBatch
+    ns = spec.n_species
+    one, library, _ = flamelet_cases(spec)
+    out = {'tolerance': tol, 'residual_scale': 'T/1e3, Y/1', 'solver': 'pseudo-transient continuation, batched'}
+    for name, make in (('one_flamelet_128pt_chi10', one), ('library_512_flamelets_64to256pt', library)):
+        rec = {}
+        sol = {}
+        for prec in (64, 32):
+            fl = make()
+            fb = FlameletBatch(fl)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            states, conv, its = fb.compute_steady_state(tolerance=tol, lu_precision=prec, raise_on_failure=False)
+            dt = time.perf_counter() - t0
+            sol[prec] = (states, conv)
+            rec['fp%d' % prec] = {'seconds': dt, 'flamelets_per_s': len(fl) / dt, 'converged': int(conv.sum()),
+                                  'flamelets': len(fl), 'iterations_mean': float(np.mean(its)),
+                                  'iterations_max': int(np.max(its)), 'counters': fb.counters,
+                                  'points': int(sum(f.n_interior_points + 2 for f in fl))}
+            print('flamelets %s fp%d: %.3f s, %d/%d converged' % (name, prec, dt, conv.sum(), len(fl)),
+                  file=sys.stderr, flush=True)
+        both = [k for k in range(len(sol[64][1])) if sol[64][1][k] and sol[32][1][k]]
+        rec['fp32_vs_fp64_max_scaled_diff'] = max((scaled_diff(sol[32][0][k], sol[64][0][k], ns) for k in both),
+                                                  default=None)
+        rec['speedup_fp32_over_fp64'] = rec['fp64']['seconds'] / rec['fp32']['seconds']
+        if name.startswith('one') and cpu:
+            from oracle.flamelet_oracle import OracleFlamelet
+            from spitfire_amd.flamelet import FlameletSpec
+            f0 = make()[0]
+            of = OracleFlamelet(FlameletSpec(mech_spec=spec, oxy_stream=f0.oxy_stream, fuel_stream=f0.fuel_stream,
+                                             grid_points=128, stoich_dissipation_rate=10.))
+            t0 = time.perf_counter()
+            st, it, ok, _ = of.steady_solve_psitc(tolerance=tol)
+            dt = time.perf_counter() - t0
+            rec['oracle'] = {'seconds': dt, 'converged': bool(ok), 'iterations': it,
+                             'fp64_vs_oracle_max_scaled_diff': scaled_diff(sol[64][0][0], st, ns) if ok else None,
+                             'fp32_vs_oracle_max_scaled_diff': scaled_diff(sol[32][0][0], st, ns) if ok else None}
+            out['cpu_baseline'] = {'value': 1. / dt, 'unit': 'flamelet solves/s', 'cores': 1, 'kind': 'port',
+                                   'sample': 'the one-flamelet case (128 points, chi_st 10) solved by the oracle: '
+                                             'the reference pseudo-transient algorithm with the FP64 CPU restatement '
+                                             'of griffon and LAPACK eigenvalues, single thread'}
+        out[name] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -60,6 +139,7 @@ def main():
     ap.add_argument('--states', type=int, default=N_STATES)
     ap.add_argument('--cpu-sample', type=int, default=131072)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-flamelets', action='store_true')
     ap.add_argument('--copy-calibration-bytes', type=int, default=0,
                     help='run one device-to-device copy of this many bytes first (counter calibration under rocprofv3)')
     args = ap.parse_args()
@@ -138,6 +218,10 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out['cpu_baseline'] = cpu_baseline(data, x, p, min(args.cpu_sample, N), threads)
+    if not args.no_flamelets:
+        del xt, pt, rhs, jac
+        torch.cuda.empty_cache()
+        out['flamelets'] = run_flamelets(spec, cpu=not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
 
 

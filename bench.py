@@ -91,6 +91,7 @@ def run_flamelets(spec, tol=1e-8, cpu=True):
     one, library, _ = flamelet_cases(spec)
     out = {'tolerance': tol, 'residual_scale': 'T/1e3, Y/1', 'solver': 'pseudo-transient continuation, batched'}
     for name, make in (('one_flamelet_128pt_chi10', one), ('library_512_flamelets_64to256pt', library)):
+        print('flamelets %s: building' % name, file=sys.stderr, flush=True)
         rec = {}
         sol = {}
         for prec in (64, 32):
@@ -98,7 +99,8 @@ def run_flamelets(spec, tol=1e-8, cpu=True):
             fb = FlameletBatch(fl)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            states, conv, its = fb.compute_steady_state(tolerance=tol, lu_precision=prec, raise_on_failure=False)
+            states, conv, its = fb.compute_steady_state(tolerance=tol, lu_precision=prec, raise_on_failure=False,
+                                                        log_rate=25)
             dt = time.perf_counter() - t0
             sol[prec] = (states, conv)
             rec['fp%d' % prec] = {'seconds': dt, 'flamelets_per_s': len(fl) / dt, 'converged': int(conv.sum()),

@@ -151,6 +151,7 @@ typedef struct gf_psitc_options {
   double max_allowed_residual;    /* 1e6    */
   double min_allowable_state_var; /* -1e-6  */
   int lu_precision;               /* 64: FP64 block LU; 32: FP32 block LU factors (FP64 residuals) */
+  int log_rate;                   /* 0: quiet; k: one progress line on stderr every k iterations */
 } gf_psitc_options;
 void gf_psitc_options_default(gf_psitc_options *opts);
 int gf_flamelet_batch_create(gf_mech *m, int nfl, const int *nzi, double pressure, const double *dz,

@@ -66,7 +66,7 @@ class PsitcOptions(ctypes.Structure):
                 ('adaptive_restart', _int), ('max_recursion_depth', _int), ('diffusion_factor', _dbl),
                 ('global_ds', _int), ('ds_safety', _dbl), ('ds_ramp', _dbl), ('ds_max', _dbl),
                 ('max_factor_line_search', _dbl), ('max_allowed_residual', _dbl), ('min_allowable_state_var', _dbl),
-                ('lu_precision', _int)]
+                ('lu_precision', _int), ('log_rate', _int)]
 
 
 _lib = None

@@ -9,7 +9,9 @@
 #include "../../include/griffon_amd.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -317,6 +319,8 @@ struct Psitc {
     std::vector<int> all;
     for (int f = 0; f < B; ++f) all.push_back(f);
     GF_TRY(start(all));
+    const auto t0 = std::chrono::steady_clock::now();
+    long sweep = 0;
 
     for (;;) {
       std::vector<int> step;
@@ -333,6 +337,14 @@ struct Psitc {
         }
       }
       if (step.empty()) break;
+      if (o.log_rate > 0 && (sweep++ % o.log_rate) == 0) {
+        double rmax = 0.;
+        for (int f : step) rmax = std::max(rmax, res[f]);
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::fprintf(stderr, "psitc sweep %ld: %zu of %d flamelets active, max residual %.3e, %.2f s\n", sweep - 1,
+                     step.size(), B, rmax, el);
+        std::fflush(stderr);
+      }
       std::vector<int> ref, first;
       for (int f : step) {
         it[f] += 1;
@@ -426,6 +438,7 @@ void gf_psitc_options_default(gf_psitc_options *o) {
   o->max_allowed_residual = 1.e6;
   o->min_allowable_state_var = -1.e-6;
   o->lu_precision = 64;
+  o->log_rate = 0;
 }
 
 int gf_flamelet_batch_create(gf_mech *m, int nfl, const int *nzi, double pressure, const double *dz,

@@ -499,7 +499,10 @@ class FlameletBatch:
                  'max_allowed_residual': 'max_allowed_residual',
                  'min_allowable_state_var': 'min_allowable_state_var'}
         for k, v in psitc_args.items():
-            if k in ('verbose', 'log_rate', 'norm_order'):
+            if k == 'log_rate':
+                o.log_rate = int(v)
+                continue
+            if k in ('verbose', 'norm_order'):
                 continue
             if k not in names:
                 raise TypeError('unknown psitc argument ' + k)

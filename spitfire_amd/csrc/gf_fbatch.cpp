@@ -58,7 +58,7 @@ struct gf_flamelet_batch {
          *eig = nullptr, *J = nullptr, *L = nullptr, *chem_rhs = nullptr, *aux = nullptr, *cpi = nullptr,
          *cmaj = nullptr, *csub = nullptr, *csup = nullptr, *mco = nullptr, *nco = nullptr, *chi = nullptr,
          *dz = nullptr, *dp = nullptr, *fval = nullptr, *fres = nullptr, *fmin = nullptr, *fdiff = nullptr;
-  float *D32 = nullptr, *L32 = nullptr;  // FP32 block-LU factors (mixed-precision path)
+  float *D32 = nullptr;  // FP32 inverse diagonal blocks (mixed-precision path); L holds the FP64 ones
   int *piv = nullptr, *dnzi = nullptr, *flag = nullptr, *gmin = nullptr, *sel = nullptr, *sys_nb = nullptr;
   long *dpt0 = nullptr, *ddof0 = nullptr, *dipt0 = nullptr, *dchi0 = nullptr, *dvoff = nullptr, *ddz0 = nullptr,
        *js_off = nullptr, *eb = nullptr, *eo = nullptr, *sys_v = nullptr, *sys_l = nullptr, *sys_p = nullptr,
@@ -97,7 +97,6 @@ size_t carve_all(gf_flamelet_batch *b, char *base) {
   b->J = c.take<double>(b->NJ);
   b->L = c.take<double>(b->NL);
   b->D32 = c.take<float>(b->NL);
-  b->L32 = c.take<float>(b->NL);
   b->chem_rhs = c.take<double>(b->NP * ns);
   b->aux = c.take<double>(b->NP * 4);
   b->cpi = c.take<double>(b->NP * ns);
@@ -261,19 +260,18 @@ struct Psitc {
     GF_TRY(launch_fb_jac_psitc(b->m, b->view, b->X, b->flag, b->ds, b->J, st));
     GF_TRY(systems(fl));
     if (o.lu_precision == 32)
-      return launch_btd_factor_t<float>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->D32, b->sys_l, b->L32,
-                                        b->sys_l, b->piv, b->sys_p, st);
-    return launch_btd_factor((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->L, b->sys_l, b->piv, b->sys_p, st);
+      return launch_btd_inv_factor<float>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->D32, b->sys_l, st);
+    return launch_btd_inv_factor<double>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->L, b->sys_l, st);
   }
 
   int solve(const std::vector<int> &fl) {
     if (fl.empty()) return GF_OK;
     GF_TRY(systems(fl));
     if (o.lu_precision == 32)
-      return launch_btd_solve_t<float>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->D32, b->sys_l, b->L32,
-                                       b->sys_l, b->piv, b->sys_p, b->R, b->dX, b->sys_x, st);
-    return launch_btd_solve((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->L, b->sys_l, b->piv, b->sys_p, b->R,
-                            b->dX, b->sys_x, st);
+      return launch_btd_inv_solve<float>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->D32, b->sys_l, b->R,
+                                         b->dX, b->sys_x, st);
+    return launch_btd_inv_solve<double>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->L, b->sys_l, b->R, b->dX,
+                                        b->sys_x, st);
   }
 
   int run(int *iterations, int *converged, double *residual) {

@@ -215,6 +215,15 @@ template <typename T>
 int launch_btd_factor_t(int nsys, const int *nblocks, int bs, double *values, const long *voff, T *dfac,
                         const long *doff, T *l_values, const long *loff, int *pivots, const long *poff,
                         hipStream_t st);
+// block Thomas with explicit diagonal-block inverses D_i^{-1} (Gauss-Jordan), stored at dinv + doff[s]
+// (bs*bs per block); the values array supplies the blocks and the off-diagonals
+template <typename T>
+int launch_btd_inv_factor(int nsys, const int *nblocks, int bs, const double *values, const long *voff, T *dinv,
+                          const long *doff, hipStream_t st);
+template <typename T>
+int launch_btd_inv_solve(int nsys, const int *nblocks, int bs, const double *values, const long *voff,
+                         const T *dinv, const long *doff, const double *rhs, double *x, const long *xoff,
+                         hipStream_t st);
 template <typename T>
 int launch_btd_solve_t(int nsys, const int *nblocks, int bs, const double *values, const long *voff, const T *dfac,
                        const long *doff, const T *l_values, const long *loff, const int *pivots, const long *poff,

@@ -42,10 +42,25 @@ enum { S_T = 0, S_RHO, S_MMW, S_CP, S_CPST, S_INVT, S_LOGT, S_CT, S_INVM, S_LOGP
 
 using ChemArgs = ChemIO;
 
+// one DPP step of the wave scan: v + (the value of lane l - shift), 0 where no such lane (old value 0)
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_add(double v) {
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+  return v + __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes, the same value returned to every lane: an inclusive scan by DPP row shifts and row
+// broadcasts (no LDS traffic), read from lane 63. The summation order is fixed, so results are reproducible.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v = dpp_add<0x111, 0xF>(v);  // row_shr:1
+  v = dpp_add<0x112, 0xF>(v);  // row_shr:2
+  v = dpp_add<0x114, 0xF>(v);  // row_shr:4
+  v = dpp_add<0x118, 0xF>(v);  // row_shr:8
+  v = dpp_add<0x142, 0xA>(v);  // row_bcast:15
+  v = dpp_add<0x143, 0xC>(v);  // row_bcast:31
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                          __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
 
 __device__ __forceinline__ double arrh(const double *k, double logT, double invT) {

@@ -83,14 +83,14 @@ def scaled_diff(a, b, ns):
     return float(np.max(np.abs(a - b) / scale))
 
 
-def run_flamelets(spec, tol=1e-8, cpu=True, library_restarts=1):
+def run_flamelets(spec, tol=1e-8, cpu=True, library_restarts=1, library_size=512):
     import numpy as np
     import torch
     from spitfire_amd.flamelet import FlameletBatch
     ns = spec.n_species
-    one, library, _ = flamelet_cases(spec)
+    one, library, _ = flamelet_cases(spec, n_library=library_size)
     out = {'tolerance': tol, 'residual_scale': 'T/1e3, Y/1', 'solver': 'pseudo-transient continuation, batched'}
-    for name, make in (('one_flamelet_128pt_chi10', one), ('library_512_flamelets_64to256pt', library)):
+    for name, make in (('one_flamelet_128pt_chi10', one), ('library_%d_flamelets_64to256pt' % library_size, library)):
         print('flamelets %s: building' % name, file=sys.stderr, flush=True)
         rec = {}
         # the library run bounds the restarts per flamelet (the reference default allows 20)
@@ -146,6 +146,7 @@ def main():
     ap.add_argument('--cpu-sample', type=int, default=131072)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-flamelets', action='store_true')
+    ap.add_argument('--library-size', type=int, default=512, help='flamelets in the library run')
     ap.add_argument('--library-restarts', type=int, default=1,
                     help='pseudo-transient restarts allowed per flamelet in the 512-flamelet library run')
     ap.add_argument('--copy-calibration-bytes', type=int, default=0,
@@ -229,7 +230,8 @@ def main():
     if not args.no_flamelets:
         del xt, pt, rhs, jac
         torch.cuda.empty_cache()
-        out['flamelets'] = run_flamelets(spec, cpu=not args.no_cpu_baseline, library_restarts=args.library_restarts)
+        out['flamelets'] = run_flamelets(spec, cpu=not args.no_cpu_baseline, library_restarts=args.library_restarts,
+                                       library_size=args.library_size)
     print(json.dumps(out), flush=True)
 
 

@@ -123,6 +123,19 @@ int gf_batch_btddod_full_solve(int nsys, const int *nblocks, int bs, const doubl
                                const double *l_values, const long *loff, const int *pivots, const long *poff,
                                const double *rhs, double *x, const long *xoff, void *stream);
 
+/* The same systems factored for throughput (the form the batched flamelet solver uses): the block Thomas
+ * recurrence of btddod_full_factorize (btddod_matrix_kernels.cpp:19-80) run from both ends of each chain at
+ * once, storing the inverse of every Schur-updated diagonal block instead of LU factors and L blocks.
+ * dinv + doff[s] receives nblocks[s] column-major bs x bs inverse blocks, as FP64 (precision 64, dinv is
+ * double *) or FP32 (precision 32, dinv is float *). The solve replaces btddod_full_solve
+ * (btddod_matrix_kernels.cpp:82-119) for these factors; right-hand sides and solutions are FP64.
+ * values are not modified; bs must be in 1..64. */
+int gf_batch_btddod_inverse_factorize(int nsys, const int *nblocks, int bs, const double *values, const long *voff,
+                                      void *dinv, const long *doff, int precision, void *stream);
+int gf_batch_btddod_inverse_solve(int nsys, const int *nblocks, int bs, const double *values, const long *voff,
+                                  const void *dinv, const long *doff, int precision, const double *rhs, double *x,
+                                  const long *xoff, void *stream);
+
 /* ---------------- many steady flamelets at once, device-resident ------------------------------------ */
 /* A batch of adiabatic flamelets sharing one mechanism and pressure, solved together on the GPU by the
  * reference's pseudo-transient continuation (Flamelet.steady_solve_psitc, flamelet.py:1487-1703): per

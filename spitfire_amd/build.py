@@ -5,12 +5,14 @@ Usage: python -m spitfire_amd.build   (or spitfire_amd.build.build())
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIBDIR = os.path.join(HERE, '_lib')
 LIB = os.path.join(LIBDIR, 'libgriffon_amd.so')
-SOURCES = ['gf_mech.cpp', 'gf_capi.cpp', 'gf_chem.hip', 'gf_btd.hip', 'gf_flamelet.hip', 'gf_eig.hip', 'gf_fbatch.cpp']
+SOURCES = ['gf_twist_4.hip', 'gf_twist_3.hip', 'gf_twist_2.hip', 'gf_twist_1.hip', 'gf_chem.hip', 'gf_btd.hip',
+           'gf_flamelet.hip', 'gf_eig.hip', 'gf_mech.cpp', 'gf_capi.cpp', 'gf_fbatch.cpp']
 ARCH = os.environ.get('GF_OFFLOAD_ARCH', 'gfx950')
 
 
@@ -21,20 +23,37 @@ def hipcc():
     raise RuntimeError('hipcc not found')
 
 
-def build(verbose=False):
-    os.makedirs(LIBDIR, exist_ok=True)
-    tmp = LIB + '.tmp'
-    cmd = [hipcc(), '--offload-arch=' + ARCH, '-O3', '-std=c++17', '-fPIC', '-shared',
+def _compile(src, obj, verbose):
+    cmd = [hipcc(), '--offload-arch=' + ARCH, '-O3', '-std=c++17', '-fPIC', '-c',
            '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',  # no packed FP32 instructions
            '-Wall', '-Wno-unused-function', '-Wno-unused-variable',
-           '-I', os.path.join(HERE, '..', 'include'),
-           '-o', tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+           '-I', os.path.join(HERE, '..', 'include'), '-o', obj, src]
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose=False, jobs=None):
+    """Compile every source to an object (in parallel), then link the shared library."""
+    objdir = os.path.join(LIBDIR, 'obj')
+    os.makedirs(objdir, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    objs = [os.path.join(objdir, os.path.splitext(s)[0] + '.o') for s in SOURCES]
+    jobs = jobs or max(1, min(len(srcs), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(lambda so: _compile(so[0], so[1], verbose), zip(srcs, objs)))
+    for src, r in zip(SOURCES, results):
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError('hipcc failed compiling ' + src)
+    tmp = LIB + '.tmp'
+    cmd = [hipcc(), '--offload-arch=' + ARCH, '-shared', '-fPIC', '-o', tmp] + objs
     if verbose:
         print(' '.join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError('hipcc failed building libgriffon_amd.so')
+        raise RuntimeError('hipcc failed linking libgriffon_amd.so')
     os.replace(tmp, LIB)
     return LIB
 

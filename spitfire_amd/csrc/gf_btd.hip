@@ -13,15 +13,22 @@
 // forward/back substitution on the identity. Solve: one wave per system, lane = row, each block staged in LDS
 // by one coalesced load, register-resident vectors, shuffles broadcast the pivot/substitution values.
 //
-// The flamelet solvers use a second form of the same recurrence (btd_inv_*): each Schur-updated diagonal
-// block is inverted by Gauss-Jordan elimination with partial pivoting and stored as D_i^{-1}, so that
-// L_i = diag(sub_{i-1}) D_{i-1}^{-1} needs no storage and both sweeps of a solve are matrix-vector products.
+// The batched flamelet solver uses a throughput form of the same recurrence (btd_inv_*): each chain is
+// eliminated from both ends at once (twisted factorisation, halving the serial chain), every Schur-updated
+// diagonal block is inverted in registers by Gauss-Jordan elimination with partial pivoting (gf_twist_impl.h)
+// and stored as its inverse, so no L blocks are stored and both sweeps of a solve are matrix-vector products.
 #include "gf_internal.h"
 #include "../../include/griffon_amd.h"
+#include "gf_wave.h"
+#include "gf_twist.h"
+
+#include <utility>
 
 namespace gfa {
 
 namespace {
+
+using wave::lane_value;
 
 constexpr int BW = 64;
 constexpr int BT_THREADS = 256;
@@ -225,206 +232,95 @@ __global__ void __launch_bounds__(BW) btd_solve_kernel(int nsys, const int *nblo
 
 // ---------------- block Thomas with explicit diagonal-block inverses ----------------
 
-// Maximum of a 32-bit unsigned value over the wave by DPP row shifts and row broadcasts (no LDS round trips).
-__device__ __forceinline__ unsigned dpp_wave_max_u32(unsigned v) {
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false));  // row_shr:1
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false));  // row_shr:2
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false));  // row_shr:4
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false));  // row_shr:8
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false));  // row_bcast:15
-  v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false));  // row_bcast:31
-  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
-}
-
-// First lane holding the largest a >= 0 among the candidate lanes (LAPACK idamax semantics); -1 if none.
-// Non-negative doubles order like their bit patterns, so the maximum is found on the high word, then on the
-// low word among the lanes that hold the maximal high word.
-__device__ __forceinline__ int wave_first_argmax(double a, bool cand) {
-  const unsigned long long bits = cand ? (unsigned long long)__double_as_longlong(a) : 0ull;
-  const unsigned hi = (unsigned)(bits >> 32), lo = (unsigned)bits;
-  const unsigned mh = dpp_wave_max_u32(hi);
-  const unsigned ml = dpp_wave_max_u32(hi == mh ? lo : 0u);
-  const unsigned long long hit = __ballot(cand && hi == mh && lo == ml);
-  return hit ? __ffsll((long long)hit) - 1 : -1;
-}
-
-// The value of x held by lane l (l uniform in the wave), through scalar registers rather than LDS.
-__device__ __forceinline__ double lane_value(double x, int l) {
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
-  const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ float lane_value(float x, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
-}
-
-// In-place Gauss-Jordan inverse of the bs x bs column-major block A in LDS with partial (row) pivoting.
-// Lane = row; wave w owns the columns w, w + 4, ... At step j every wave reads the pivot column j into
-// registers, finds the pivot (the same in every wave), and then swaps, scales and updates its own columns
-// entirely in registers, so the only barrier per step is the one that publishes the updated columns. The row
-// interchanges become column interchanges of the inverse, returned in cols: column k of the inverse is
-// column cols[k] of A.
+// y = D v for a column-major bs x bs block and v held one entry per lane (lane = row); the entries of v are
+// broadcast through scalar registers and four partial sums keep the column loads independent.
 template <typename T>
-__device__ void lds_gj_inverse(T *A, int bs, int *cols, int *swp) {
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  constexpr int NW = BT_THREADS / 64;
+__device__ __forceinline__ double wave_matvec(const T *D, int bs, double v, int lane) {
   const bool act = lane < bs;
-  for (int j = 0; j < bs; ++j) {
-    T f = act ? A[j * bs + lane] : T(0);  // the pivot column, lane = row
-    const int idx = wave_first_argmax(fabs((double)f), act && lane >= j);
-    const int p = idx < 0 ? j : idx;
-    if (t == 0) swp[j] = p;
-    // pivot column after the interchange of rows j and p
-    const T fj = lane_value(f, j), fp = lane_value(f, p);
-    if (lane == j) f = fp;
-    else if (lane == p) f = fj;
-    const T ip = T(1) / fp;
-    const T fr = (lane == j) ? T(0) : f;
-    // this wave's columns, all loaded before any is used so that their shuffles overlap
-    constexpr int U = (64 + NW - 1) / NW;
-    T x[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int c = wave + u * NW;
-      x[u] = (act && c < bs) ? A[c * bs + lane] : T(0);
-    }
-    __syncthreads();  // every wave has read column j and its own columns before any column is rewritten
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int c = wave + u * NW;
-      if (c < bs) {  // uniform in the wave
-        const T xj = lane_value(x[u], j), xp = lane_value(x[u], p);
-        const T xr = (lane == j) ? xp : ((lane == p) ? xj : x[u]);  // rows j and p interchanged
-        const T rj = (c == j) ? ip : xp * ip;                      // scaled pivot row entry
-        x[u] = (lane == j) ? rj : ((c == j) ? -fr * ip : xr - fr * rj);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int c = wave + u * NW;
-      if (act && c < bs) A[c * bs + lane] = x[u];
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    for (int k = 0; k < bs; ++k) cols[k] = k;
-    for (int k = bs - 1; k >= 0; --k) {
-      const int q = swp[k];
-      const int a0 = cols[k];
-      cols[k] = cols[q];
-      cols[q] = a0;
-    }
-  }
-  __syncthreads();
-}
-
-// D_0 = V_0; D_i = V_i - diag(sub_{i-1}) D_{i-1}^{-1} diag(sup_{i-1}); stores D_i^{-1} at dinv + doff[sys].
-template <typename T>
-__global__ void __launch_bounds__(BT_THREADS) btd_inv_factor_kernel(const int *nblocks, int bs, const double *values,
-                                                                   const long *voff, T *dinv, const long *doff) {
-  extern __shared__ double sm_raw[];
-  T *A = reinterpret_cast<T *>(sm_raw);
-  int *cols = reinterpret_cast<int *>(A + bs * bs);
-  int *swp = cols + 64;
-  const int sys = blockIdx.x;
-  const int nb = nblocks[sys];
-  const int nb2 = bs * bs;
-  const double *V = values + voff[sys];
-  T *Dg = dinv + doff[sys];
-  const double *sub = V + (long)nb * nb2;
-  const double *sup = sub + (long)(nb - 1) * bs;
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;
-  constexpr int NW = BT_THREADS / 64;
-  for (int i = 0; i < nb; ++i) {
-    const long o = (long)i * nb2;
-    if (lane < bs) {
-      if (i == 0) {
-        for (int c = wave; c < bs; c += NW) A[c * bs + lane] = (T)V[c * bs + lane];
-      }
-    }
-    if (i > 0) {  // Schur update from D_{i-1}^{-1}: column c of the inverse is column cols[c] of A
-      const int b1 = (i - 1) * bs;
-      const double sr = lane < bs ? sub[b1 + lane] : 0.;
-      T src[(64 + NW - 1) / NW];
-#pragma unroll
-      for (int u = 0; u < (64 + NW - 1) / NW; ++u) {
-        const int c = wave + u * NW;
-        src[u] = (lane < bs && c < bs) ? A[cols[c] * bs + lane] : T(0);
-      }
-      __syncthreads();
-#pragma unroll
-      for (int u = 0; u < (64 + NW - 1) / NW; ++u) {
-        const int c = wave + u * NW;
-        if (lane < bs && c < bs) {
-          const int e = c * bs + lane;
-          const T l = src[u] * (T)sr;
-          A[e] = (T)(V[o + e] + (-sup[b1 + c]) * (double)l);
-        }
-      }
-    }
-    __syncthreads();
-    lds_gj_inverse(A, bs, cols, swp);
-    if (lane < bs)
-      for (int k = wave; k < bs; k += NW) Dg[o + k * bs + lane] = A[cols[k] * bs + lane];
-  }
-}
-
-// One wave per system, lane = row: y_i = b_i - sub_{i-1} * (D_{i-1}^{-1} y_{i-1}), then
-// x_i = D_i^{-1} (y_i - sup_i * x_{i+1}); the vector is broadcast from LDS, the inverse block is read by
-// coalesced column loads, and four partial sums keep the loads independent.
-template <typename T>
-__device__ __forceinline__ double wave_matvec(const T *D, int bs, const double *vec, int lane) {
   double m0 = 0., m1 = 0., m2 = 0., m3 = 0.;
   int c = 0;
   for (; c + 3 < bs; c += 4) {
-    m0 += (double)D[c * bs + lane] * vec[c];
-    m1 += (double)D[(c + 1) * bs + lane] * vec[c + 1];
-    m2 += (double)D[(c + 2) * bs + lane] * vec[c + 2];
-    m3 += (double)D[(c + 3) * bs + lane] * vec[c + 3];
+    const double v0 = lane_value(v, c), v1 = lane_value(v, c + 1), v2 = lane_value(v, c + 2),
+                 v3 = lane_value(v, c + 3);
+    if (act) {
+      m0 += (double)D[c * bs + lane] * v0;
+      m1 += (double)D[(c + 1) * bs + lane] * v1;
+      m2 += (double)D[(c + 2) * bs + lane] * v2;
+      m3 += (double)D[(c + 3) * bs + lane] * v3;
+    }
   }
-  for (; c < bs; ++c) m0 += (double)D[c * bs + lane] * vec[c];
+  for (; c < bs; ++c) {
+    const double v0 = lane_value(v, c);
+    if (act) m0 += (double)D[c * bs + lane] * v0;
+  }
   return (m0 + m1) + (m2 + m3);
 }
 
+// Solve with the twisted factors, one workgroup of two waves per system, lane = row:
+//   wave 0:  y_0 = b_0,  y_i = b_i - sub_{i-1} (D_{i-1}^{-1} y_{i-1}),  i < m
+//   wave 1:  y_{n-1} = b_{n-1},  y_i = b_i - sup_i (E_{i+1}^{-1} y_{i+1}),  i > m
+//   middle:  x_m = C_m^{-1} (b_m - sub_{m-1} (D_{m-1}^{-1} y_{m-1}) - sup_m (E_{m+1}^{-1} y_{m+1}))
+//   wave 0:  x_i = D_i^{-1} (y_i - sup_i x_{i+1}),  i = m-1 .. 0;   wave 1:  x_i = E_i^{-1} (y_i - sub_{i-1} x_{i-1}),  i > m
 template <typename T>
-__global__ void __launch_bounds__(BW) btd_inv_solve_kernel(int nsys, const int *nblocks, int bs, const double *values,
-                                                          const long *voff, const T *dinv, const long *doff,
-                                                          const double *rhs, double *x, const long *xoff) {
-  __shared__ double vec[64];
+__global__ void __launch_bounds__(128) btd_twist_solve_kernel(int nsys, const int *nblocks, int bs,
+                                                              const double *values, const long *voff,
+                                                              const T *dinv, const long *doff, const double *rhs,
+                                                              double *x, const long *xoff) {
+  __shared__ double mid[2][64];
   const int sys = blockIdx.x;
   if (sys >= nsys) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool top = threadIdx.x < 64;
+  const bool act = lane < bs;
   const int nb = nblocks[sys];
-  const int nb2 = bs * bs;
+  const int m = (nb - 1) / 2;
+  const long nb2 = (long)bs * bs;
   const double *V = values + voff[sys];
   const T *DI = dinv + doff[sys];
   const double *b = rhs + xoff[sys];
   double *X = x + xoff[sys];
   const double *sub = V + (long)nb * nb2;
   const double *sup = sub + (long)(nb - 1) * bs;
-  const bool act = lane < bs;
 
-  double y = act ? b[lane] : 0.;
-  if (act) X[lane] = y;
-  for (int i = 1; i < nb; ++i) {
-    vec[lane] = y;
-    __syncthreads();
-    const double m = act ? wave_matvec(DI + (long)(i - 1) * nb2, bs, vec, lane) : 0.;
-    y = act ? b[(long)i * bs + lane] - sub[(long)(i - 1) * bs + lane] * m : 0.;
+  // sweep towards the middle; y_i is kept in X
+  const int len = top ? m : nb - 1 - m;
+  double y = 0.;
+  for (int t = 0; t < len; ++t) {
+    const int i = top ? t : nb - 1 - t;
+    const double bi = act ? b[(long)i * bs + lane] : 0.;
+    if (t == 0) {
+      y = bi;
+    } else {
+      const int prev = top ? i - 1 : i + 1;
+      const double mv = wave_matvec(DI + prev * nb2, bs, y, lane);
+      const double cpl = act ? (top ? sub[(long)(i - 1) * bs + lane] : sup[(long)i * bs + lane]) : 0.;
+      y = bi - cpl * mv;
+    }
     if (act) X[(long)i * bs + lane] = y;
-    __syncthreads();
   }
-  double xn = 0.;
-  for (int i = nb - 1; i >= 0; --i) {
-    double r = act ? X[(long)i * bs + lane] : 0.;
-    if (i < nb - 1 && act) r = r - sup[(long)i * bs + lane] * xn;
-    vec[lane] = r;
-    __syncthreads();
-    xn = act ? wave_matvec(DI + (long)i * nb2, bs, vec, lane) : 0.;
+  // this wave's term of the middle equation
+  double term = 0.;
+  if (len > 0) {
+    const int prev = top ? m - 1 : m + 1;
+    const double mv = wave_matvec(DI + prev * nb2, bs, y, lane);
+    term = act ? (top ? sub[(long)(m - 1) * bs + lane] : sup[(long)m * bs + lane]) * mv : 0.;
+  }
+  mid[top ? 0 : 1][lane] = term;
+  __syncthreads();
+  double xm = 0.;
+  {
+    const double r = act ? b[(long)m * bs + lane] - mid[0][lane] - mid[1][lane] : 0.;
+    xm = wave_matvec(DI + m * nb2, bs, r, lane);  // both waves form x_m (no second exchange)
+  }
+  if (top && act) X[(long)m * bs + lane] = xm;
+  // sweep outwards
+  double xn = xm;
+  for (int t = len - 1; t >= 0; --t) {
+    const int i = top ? t : nb - 1 - t;
+    const double yi = act ? X[(long)i * bs + lane] : 0.;
+    const double cpl = act ? (top ? sup[(long)i * bs + lane] : sub[(long)(i - 1) * bs + lane]) : 0.;
+    xn = wave_matvec(DI + i * nb2, bs, yi - cpl * xn, lane);
     if (act) X[(long)i * bs + lane] = xn;
-    __syncthreads();
   }
 }
 
@@ -473,16 +369,20 @@ int launch_btd_solve_t(int nsys, const int *nblocks, int bs, const double *value
 }
 
 template <typename T>
+using TwistFactorFn = void (*)(int, const int *, const double *, const long *, T *, const long *, hipStream_t);
+
+template <typename T, int... I>
+constexpr TwistFactorFn<T> twist_factor_table_entry(int bs, std::integer_sequence<int, I...>) {
+  constexpr TwistFactorFn<T> tab[] = {&launch_twist_factor<T, I + 1>...};
+  return tab[bs - 1];
+}
+
+template <typename T>
 int launch_btd_inv_factor(int nsys, const int *nblocks, int bs, const double *values, const long *voff, T *dinv,
                           const long *doff, hipStream_t st) {
   if (nsys <= 0) return GF_OK;
   if (bs < 1 || bs > 64) return fail(GF_ERR_UNSUPPORTED, "block size must be in 1..64");
-  const size_t lds = sizeof(T) * bs * bs + sizeof(int) * 128 + 16;
-  auto kern = btd_inv_factor_kernel<T>;
-  if (lds > 65536)
-    if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-      return fail(GF_ERR_UNSUPPORTED, "block too large for LDS");
-  hipLaunchKernelGGL(kern, dim3(nsys), dim3(BT_THREADS), lds, st, nblocks, bs, values, voff, dinv, doff);
+  twist_factor_table_entry<T>(bs, std::make_integer_sequence<int, 64>{})(nsys, nblocks, values, voff, dinv, doff, st);
   if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, "btd inverse factor launch failed");
   return GF_OK;
 }
@@ -493,7 +393,7 @@ int launch_btd_inv_solve(int nsys, const int *nblocks, int bs, const double *val
                          hipStream_t st) {
   if (nsys <= 0) return GF_OK;
   if (bs < 1 || bs > 64) return fail(GF_ERR_UNSUPPORTED, "block size must be in 1..64");
-  hipLaunchKernelGGL(btd_inv_solve_kernel<T>, dim3(nsys), dim3(BW), 0, st, nsys, nblocks, bs, values, voff, dinv,
+  hipLaunchKernelGGL(btd_twist_solve_kernel<T>, dim3(nsys), dim3(128), 0, st, nsys, nblocks, bs, values, voff, dinv,
                      doff, rhs, x, xoff);
   if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, "btd inverse solve launch failed");
   return GF_OK;

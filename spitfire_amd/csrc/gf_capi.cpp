@@ -413,4 +413,27 @@ int gf_batch_btddod_full_solve(int nsys, const int *nblocks, int bs, const doubl
                           static_cast<hipStream_t>(stream));
 }
 
+int gf_batch_btddod_inverse_factorize(int nsys, const int *nblocks, int bs, const double *values, const long *voff,
+                                      void *dinv, const long *doff, int precision, void *stream) {
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (precision == 64)
+    return launch_btd_inv_factor<double>(nsys, nblocks, bs, values, voff, static_cast<double *>(dinv), doff, st);
+  if (precision == 32)
+    return launch_btd_inv_factor<float>(nsys, nblocks, bs, values, voff, static_cast<float *>(dinv), doff, st);
+  return fail(GF_ERR_ARG, "precision must be 64 or 32");
+}
+
+int gf_batch_btddod_inverse_solve(int nsys, const int *nblocks, int bs, const double *values, const long *voff,
+                                  const void *dinv, const long *doff, int precision, const double *rhs, double *x,
+                                  const long *xoff, void *stream) {
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (precision == 64)
+    return launch_btd_inv_solve<double>(nsys, nblocks, bs, values, voff, static_cast<const double *>(dinv), doff,
+                                        rhs, x, xoff, st);
+  if (precision == 32)
+    return launch_btd_inv_solve<float>(nsys, nblocks, bs, values, voff, static_cast<const float *>(dinv), doff, rhs,
+                                       x, xoff, st);
+  return fail(GF_ERR_ARG, "precision must be 64 or 32");
+}
+
 }  // extern "C"

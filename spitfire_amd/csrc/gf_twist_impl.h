@@ -1,0 +1,159 @@
+// Kernels behind launch_twist_factor; included only by the gf_twist_*.hip units, each of which instantiates
+// a range of block sizes with GF_TWIST(N).
+#pragma once
+#include "gf_twist.h"
+#include "gf_wave.h"
+
+namespace gfa {
+
+namespace {
+
+using wave::lane_value;
+using wave::wave_first_argmax;
+
+// Gauss-Jordan inverse of an N x N block held in the registers of one wave, with partial (row) pivoting.
+//
+// Layout: lane l holds one row, x[k] the entry in slot k. The entry of lane l, slot k is element
+// (rowid of lane l, colid of slot k) of the logical matrix, where colid of slot k is held by lane k of
+// `colid`. Rows are never interchanged: the pivot of step j is the unused lane with the largest |entry| in
+// the pivot column (ties: lowest lane), and the exchange at (pivot lane p, slot j) is done in place. After N
+// steps the block holds the inverse in the same kind of layout: the lane pivoted at step j holds row
+// colid_j of the inverse, slot j holds its column rowid_{p_j}.
+//
+// The slots form a queue: the pivot column is always x[0]; each step shifts the other columns down by one
+// and appends the transformed pivot column at x[N-1], so that all register indices are compile-time
+// constants and after N steps every column is back in its slot. The pivot row is broadcast through scalar
+// registers (readlane), so a step has no LDS traffic and no barrier.
+template <typename T, int N>
+__device__ __forceinline__ void reg_gj_inverse(T (&x)[N], int &rowid, int &colid, int lane) {
+  bool used = lane >= N;
+  for (int j = 0; j < N; ++j) {
+    const T f = x[0];
+    const int p = wave_first_argmax(fabs(f), !used);  // N - j unused lanes remain, so p >= 0
+    const T ipiv = T(1) / lane_value(f, p);
+    const bool isp = lane == p;
+    const T g = isp ? -ipiv : f * ipiv;   // multiplier of the broadcast pivot row
+    const T keep = isp ? T(0) : T(1);     // the pivot row itself becomes (pivot row) / pivot
+#pragma unroll
+    for (int c = 1; c < N; ++c) x[c - 1] = fma(-g, lane_value(x[c], p), x[c] * keep);
+    x[N - 1] = -g;                        // the pivot column: -f / pivot, and 1 / pivot on the pivot lane
+    const int cj = __builtin_amdgcn_readlane(colid, j);
+    const int rp = __builtin_amdgcn_readlane(rowid, p);
+    if (isp) {
+      rowid = cj;
+      used = true;
+    }
+    if (lane == j) colid = rp;
+  }
+}
+
+// Block i of the values in the identity layout (lane = row, slot = column).
+template <typename T, int N>
+__device__ __forceinline__ void reg_load_block(T (&x)[N], const double *Vi, int &rowid, int &colid, int lane) {
+  rowid = lane;
+  colid = lane;
+#pragma unroll
+  for (int k = 0; k < N; ++k) x[k] = lane < N ? (T)Vi[k * N + lane] : T(0);
+}
+
+// x <- V_i - diag(rs) X diag(cs), where x holds X (an inverse from the previous block of the chain) in its
+// permuted layout; the result keeps that layout.
+template <typename T, int N>
+__device__ __forceinline__ void reg_schur(T (&x)[N], const double *Vi, const double *rs, const double *cs,
+                                          int rowid, int colid, int lane) {
+  const bool act = lane < N;
+  const double r = act ? rs[rowid] : 0.;
+  const double csl = act ? cs[lane] : 0.;  // lane q holds cs[q]
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int c = __builtin_amdgcn_readlane(colid, k);
+    const double cc = lane_value(csl, c);
+    if (act) x[k] = (T)(Vi[c * N + rowid] - r * cc * (double)x[k]);
+  }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void reg_store_block(const T (&x)[N], T *Di, int rowid, int colid, int lane) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int c = __builtin_amdgcn_readlane(colid, k);
+    if (lane < N) Di[c * N + rowid] = x[k];
+  }
+}
+
+// Twisted (burn-at-both-ends) block factorisation, one workgroup of two waves per system. Row i of the
+// system is diag(sub_{i-1}) x_{i-1} + V_i x_i + diag(sup_i) x_{i+1} = b_i; with m = (n - 1) / 2
+//   wave 0:  D_0 = V_0,      D_i = V_i - diag(sub_{i-1}) D_{i-1}^{-1} diag(sup_{i-1}),  i = 1 .. m-1
+//   wave 1:  E_{n-1} = V_{n-1},  E_i = V_i - diag(sup_i) E_{i+1}^{-1} diag(sub_i),      i = n-2 .. m+1
+//   middle:  C_m = V_m - diag(sub_{m-1}) D_{m-1}^{-1} diag(sup_{m-1}) - diag(sup_m) E_{m+1}^{-1} diag(sub_m)
+// and block i of dinv receives D_i^{-1} (i < m), C_m^{-1} (i = m) or E_i^{-1} (i > m), column-major. The top
+// half is the reference's forward block Thomas recurrence; running the bottom half at the same time halves
+// the serial chain. The bottom wave hands its term of C_m to the top wave through LDS.
+template <typename T, int N>
+__global__ void __launch_bounds__(128) btd_twist_factor_kernel(const int *nblocks, const double *values,
+                                                               const long *voff, T *dinv, const long *doff) {
+  __shared__ double sb[N * N];
+  constexpr long NB2 = (long)N * N;
+  const int sys = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool top = threadIdx.x < 64;
+  const int nb = nblocks[sys];
+  const int m = (nb - 1) / 2;
+  const double *V = values + voff[sys];
+  const double *sub = V + nb * NB2;
+  const double *sup = sub + (long)(nb - 1) * N;
+  T *Dg = dinv + doff[sys];
+  T x[N];
+  int rowid = lane, colid = lane;
+  const int len = top ? m : nb - 1 - m;
+  for (int t = 0; t < len; ++t) {
+    const int i = top ? t : nb - 1 - t;
+    if (t == 0) reg_load_block<T, N>(x, V + i * NB2, rowid, colid, lane);
+    else if (top) reg_schur<T, N>(x, V + i * NB2, sub + (i - 1) * N, sup + (i - 1) * N, rowid, colid, lane);
+    else reg_schur<T, N>(x, V + i * NB2, sup + i * N, sub + i * N, rowid, colid, lane);
+    reg_gj_inverse<T, N>(x, rowid, colid, lane);
+    reg_store_block<T, N>(x, Dg + i * NB2, rowid, colid, lane);
+  }
+  const bool below = nb - 1 - m > 0;
+  if (!top && below) {
+    const double r = lane < N ? sup[m * N + rowid] : 0.;
+    const double csl = lane < N ? sub[m * N + lane] : 0.;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const int c = __builtin_amdgcn_readlane(colid, k);
+      const double cc = lane_value(csl, c);
+      if (lane < N) sb[c * N + rowid] = r * (double)x[k] * cc;
+    }
+  }
+  __syncthreads();
+  if (top) {
+    if (m == 0) reg_load_block<T, N>(x, V + m * NB2, rowid, colid, lane);
+    else reg_schur<T, N>(x, V + m * NB2, sub + (m - 1) * N, sup + (m - 1) * N, rowid, colid, lane);
+    if (below) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const int c = __builtin_amdgcn_readlane(colid, k);
+        if (lane < N) x[k] = (T)((double)x[k] - sb[c * N + rowid]);
+      }
+    }
+    reg_gj_inverse<T, N>(x, rowid, colid, lane);
+    reg_store_block<T, N>(x, Dg + m * NB2, rowid, colid, lane);
+  }
+}
+
+}  // namespace
+
+template <typename T, int N>
+void launch_twist_factor(int nsys, const int *nblocks, const double *values, const long *voff, T *dinv,
+                         const long *doff, hipStream_t st) {
+  hipLaunchKernelGGL((btd_twist_factor_kernel<T, N>), dim3(nsys), dim3(128), 0, st, nblocks, values, voff, dinv, doff);
+}
+
+}  // namespace gfa
+
+// Explicit instantiations of the launcher (and so of its kernel) for block size N, FP64 and FP32 inverses.
+#define GF_TWIST(N)                                                                                        \
+  template void gfa::launch_twist_factor<double, N>(int, const int *, const double *, const long *, double *, \
+                                                    const long *, hipStream_t);                            \
+  template void gfa::launch_twist_factor<float, N>(int, const int *, const double *, const long *, float *,   \
+                                                   const long *, hipStream_t);

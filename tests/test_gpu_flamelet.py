@@ -133,6 +133,57 @@ def test_batched_btd_ragged_matches_oracle(mech_53_data, oracle_lib):
         assert e <= 1e-11
 
 
+@pytest.mark.parametrize('bs', [1, 5, 53, 64])
+@pytest.mark.parametrize('precision', [64, 32])
+def test_batched_btd_inverse_path_matches_oracle(bs, precision, mech_53_data, oracle_lib):
+    """The throughput form of the block Thomas recurrence (inverse diagonal blocks, chain eliminated from both
+    ends): ragged systems of 1..255 blocks, odd and even, against the oracle's block Thomas with LAPACK-style
+    LU. FP64 factors agree to 1e-11 of max|x|; FP32 factors (FP64 vectors) to 1e-4, the rounding of FP32
+    inverse blocks times the condition of these diagonally weighted systems."""
+    import torch
+    rng = np.random.default_rng(11 + bs)
+    sizes = [1, 2, 3, 4, 7, 64, 255] if bs <= 53 else [1, 2, 5, 64]
+    vals, rhs = [], []
+    for nb in sizes:
+        v = rng.standard_normal(bs * (nb * bs + 2 * (nb - 1)))
+        for i in range(nb):
+            v[i * bs * bs:(i + 1) * bs * bs].reshape(bs, bs)[np.diag_indices(bs)] += 10. + 4. * np.sqrt(bs)
+        vals.append(v)
+        rhs.append(rng.standard_normal(nb * bs))
+    voff = np.cumsum([0] + [v.size for v in vals])[:-1]
+    doff = np.cumsum([0] + [nb * bs * bs for nb in sizes])[:-1]
+    xoff = np.cumsum([0] + [nb * bs for nb in sizes])[:-1]
+    dev = torch.device('cuda:0')
+    V = torch.tensor(np.concatenate(vals), device=dev)
+    D = torch.zeros(int(sum(nb * bs * bs for nb in sizes)), dtype=torch.float64 if precision == 64 else torch.float32,
+                    device=dev)
+    B = torch.tensor(np.concatenate(rhs), device=dev)
+    X = torch.zeros_like(B)
+    nbt = torch.tensor(sizes, dtype=torch.int32, device=dev)
+    vo, do, xo = [torch.tensor(a.astype(np.int64), device=dev) for a in (voff, doff, xoff)]
+    from spitfire_amd import _native
+    lib = _native.lib()
+    s = torch.cuda.current_stream().cuda_stream
+    say('inverse btd bs', bs, 'precision', precision, 'sizes', sizes)
+    _native.check(lib.gf_batch_btddod_inverse_factorize(len(sizes), nbt.data_ptr(), bs, V.data_ptr(), vo.data_ptr(),
+                                                        D.data_ptr(), do.data_ptr(), precision, s))
+    _native.check(lib.gf_batch_btddod_inverse_solve(len(sizes), nbt.data_ptr(), bs, V.data_ptr(), vo.data_ptr(),
+                                                    D.data_ptr(), do.data_ptr(), precision, B.data_ptr(),
+                                                    X.data_ptr(), xo.data_ptr(), s))
+    torch.cuda.synchronize()
+    Xh = X.cpu().numpy()
+    assert np.array_equal(V.cpu().numpy(), np.concatenate(vals))  # the values are not modified
+    from spitfire_amd.mechdata import CompiledMechanism
+    cpu = oracle_lib.CpuGriffon(CompiledMechanism(mech_53_data))
+    tol = 1e-11 if precision == 64 else 1e-4
+    for k, nb in enumerate(sizes):
+        d, l, piv = cpu.btddod_full_factorize(vals[k], nb, bs)
+        xr = cpu.btddod_full_solve(d, l, piv, rhs[k], nb, bs)
+        e = np.max(np.abs(Xh[xoff[k]:xoff[k] + nb * bs] - xr)) / np.max(np.abs(xr))
+        say('system', nb, 'blocks: err %.3e' % e)
+        assert e <= tol
+
+
 @pytest.mark.parametrize('fx, mech', [('flamelet_c1.npz', 'mech_c1_data'), ('flamelet_53.npz', 'mech_53_data')])
 def test_flamelet_eigenvalue_limiter_matches_reference(fx, mech, request, oracle_lib):
     """expeig = max(0, max Re(lambda(chemistry block)) - diffterm) per point (flamelet_kernels.cpp:1329-1341):

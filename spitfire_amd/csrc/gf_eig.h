@@ -161,10 +161,10 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld,
         p = (rr * ss - ww) / a[(m + 1) * ld + m] + a[m * ld + m + 1];
         q = a[(m + 1) * ld + m + 1] - z - rr - ss;
         r = a[(m + 2) * ld + m + 1];
-        const double s = fabs(p) + fabs(q) + fabs(r);
-        p /= s;
-        q /= s;
-        r /= s;
+        const double is = 1. / (fabs(p) + fabs(q) + fabs(r));
+        p *= is;
+        q *= is;
+        r *= is;
         if (m == l) break;
         const double u = fabs(a[m * ld + m - 1]) * (fabs(q) + fabs(r));
         const double v = fabs(p) * (fabs(a[(m - 1) * ld + m - 1]) + fabs(z) + fabs(a[(m + 1) * ld + m + 1]));
@@ -183,16 +183,18 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld,
           r = k != nn - 1 ? a[(k + 2) * ld + k - 1] : 0.;
           xk = fabs(p) + fabs(q) + fabs(r);
           if (xk != 0.) {
-            p /= xk;
-            q /= xk;
-            r /= xk;
+            const double ix = 1. / xk;
+            p *= ix;
+            q *= ix;
+            r *= ix;
           }
         }
         const double sn = sqrt(p * p + q * q + r * r);
         const double s = p >= 0. ? sn : -sn;
         if (s == 0.) continue;
-        const double hx = (p + s) / s, hy = q / s, hz = r / s;
-        const double qq = q / (p + s), rr = r / (p + s);
+        const double is = 1. / s, ips = 1. / (p + s);
+        const double hx = (p + s) * is, hy = q * is, hz = r * is;
+        const double qq = q * ips, rr = r * ips;
         const bool three = k != nn - 1;
         const double sub = (k == m) ? (l != m ? -a[k * ld + k - 1] : a[k * ld + (k > 0 ? k - 1 : 0)])
                                     : -s * xk;

@@ -1,6 +1,7 @@
 // Per-block largest-real-part eigenvalue kernel (the pseudo-time limiter input of the flamelet solvers).
 #include "gf_internal.h"
 #include "gf_eig.h"
+#include "gf_wave.h"
 #include "../../include/griffon_amd.h"
 
 namespace gfa {
@@ -13,9 +14,8 @@ struct WaveLanes {
     __syncthreads();
   }
   template <class F> __device__ double sum(int lo, int hi, F f) {
-    double v = (lane >= lo && lane < hi) ? f(lane) : 0.;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);  // identical in every lane
-    return v;
+    // DPP scan (no LDS permutes); the same value in every lane
+    return wave::dpp_wave_sum((lane >= lo && lane < hi) ? f(lane) : 0.);
   }
 };
 

@@ -17,6 +17,25 @@ __device__ __forceinline__ unsigned dpp_wave_max_u32(unsigned v) {
   return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
+// Sum of a double over the 64 lanes, returned to every lane: an inclusive scan by DPP row shifts and row
+// broadcasts, read from lane 63. The summation order is fixed, so results are reproducible.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_add_f64(double v) {
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+  return v + __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double dpp_wave_sum(double v) {
+  v = dpp_add_f64<0x111, 0xF>(v);  // row_shr:1
+  v = dpp_add_f64<0x112, 0xF>(v);  // row_shr:2
+  v = dpp_add_f64<0x114, 0xF>(v);  // row_shr:4
+  v = dpp_add_f64<0x118, 0xF>(v);  // row_shr:8
+  v = dpp_add_f64<0x142, 0xA>(v);  // row_bcast:15
+  v = dpp_add_f64<0x143, 0xC>(v);  // row_bcast:31
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                          __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+
 // First lane holding the largest a >= 0 among the candidate lanes (LAPACK idamax semantics); -1 if none.
 // Non-negative doubles order like their bit patterns, so the maximum is found on the high word, then on the
 // low word among the lanes that hold the maximal high word.

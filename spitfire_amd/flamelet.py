@@ -79,8 +79,19 @@ class FlameletSpec:
         self.sensitivity_transform_type = sensitivity_transform_type
 
 
+class FlameletHistory(dict):
+    """Fields of a transient flamelet run, each an array [time, mixture fraction] including the boundary
+    streams (the reference returns a spitfire Library with these dimensions, flamelet.py:1187-1211)."""
+
+    def __init__(self, t, z):
+        super().__init__()
+        self.time_values = np.asarray(t)
+        self.mixture_fraction_values = np.asarray(z)
+
+
 class Flamelet:
-    """An adiabatic steady flamelet solved with the griffon drop-in (flamelet.py:269-1778)."""
+    """An adiabatic flamelet on the griffon drop-in: steady solvers and transient integration
+    (flamelet.py:269-1778)."""
 
     def __init__(self, flamelet_specs=None, **kwargs):
         fs = flamelet_specs if isinstance(flamelet_specs, FlameletSpec) else FlameletSpec(**(flamelet_specs or kwargs))
@@ -139,6 +150,8 @@ class Flamelet:
         self._block_thomas_l_values = np.zeros(n * n * nzi)
         self._block_thomas_d_pivots = np.zeros(self._jac_nelements_griffon, dtype=np.int32)
         self._iteration_count = None
+        self._current_time = 0.
+        self._jacobian_values = None
 
     def _setup_stencils(self):
         self._griffon.flamelet_stencils(self._dz, self._nz_interior, self._x, 1. / self._lewis_numbers,
@@ -352,19 +365,92 @@ class Flamelet:
         return state, iteration_count, True, np.min(ds)
 
     def compute_steady_state(self, tolerance=1.e-6, verbose=False, use_psitc=True, newton_args=None,
-                             psitc_args=None):
+                             psitc_args=None, transient_args=None):
+        """Newton, then pseudo-transient continuation, then ESDIRK64 time stepping to steady state
+        (flamelet.py:1705-1778); returns the converged interior state."""
         nargs = {'tolerance': tolerance, 'verbose': verbose, 'max_iterations': 38}
         nargs.update(newton_args or {})
         state, it, conv = self.steady_solve_newton(**nargs)
         if conv:
             return state
+        mds = 1.e-6
         if use_psitc:
             pargs = {'tolerance': tolerance, 'verbose': verbose, 'max_iterations': 400}
             pargs.update(psitc_args or {})
-            state, it, conv, _ = self.steady_solve_psitc(**pargs)
+            state, it, conv, mds = self.steady_solve_psitc(**pargs)
             if conv:
                 return state
-        raise RuntimeError('steady flamelet solve did not converge')
+        targs = {'steady_tolerance': tolerance, 'transient_tolerance': 1.e-8, 'max_time_step': 1.e4,
+                 'first_time_step': 1.e-2 * mds, 'maximum_steps_per_jacobian': 10, 'save_first_and_last_only': True}
+        targs.update(transient_args or {})
+        self.integrate_to_steady(**targs)
+        return self.current_interior_state
+
+    # ---------------- transient integration (flamelet.py:1046-1237) ----------------
+    def _setup_block_thomas(self, t, state_interior, prefactor):
+        """prefactor * J - I, block-Thomas factored in place (flamelet.py:756-762)"""
+        self._jacobian_values = self._adiabatic_jac(state_interior, scale_and_offset=True, prefactor=prefactor)
+        self._factorize(self._jacobian_values)
+
+    def _solve_block_thomas(self, residual):
+        return self._solve(self._jacobian_values, residual)
+
+    def integrate(self, stop_at_time=None, stop_at_steady=None, stop_criteria=None, first_time_step=1.e-6,
+                  max_time_step=1.e-3, minimum_time_step_count=40, transient_tolerance=1.e-10,
+                  maximum_steps_per_jacobian=10, nonlinear_solve_tolerance=1.e-12, save_first_and_last_only=False,
+                  return_info=False):
+        """ESDIRK64 with a frozen-Jacobian Newton solver, PI step control and block-Thomas linear solves of
+        dt*gamma*J - I (flamelet.py:1046-1213, linear_solver='block thomas'); negative entries are clipped after
+        every step. Starts from the current state and time and advances them. Returns a FlameletHistory with
+        the time values and, per field, an array [time, mixture fraction] that includes the boundary streams."""
+        from .time_integration import KennedyCarpenterS6P4Q3, PIController, SimpleNewtonSolver, odesolve
+        w = 1. / self._variable_scales
+        newton = SimpleNewtonSolver(evaluate_jacobian_every_iter=False, norm_weighting=w,
+                                    tolerance=nonlinear_solve_tolerance)
+        stepper = KennedyCarpenterS6P4Q3(newton, norm_weighting=w)
+        control = PIController(first_step=first_time_step, max_step=max_time_step, target_error=transient_tolerance)
+
+        def post_step(t, state, *args):
+            state[state < 0.] = 0.
+            return state
+
+        out = odesolve(self._adiabatic_rhs, self._current_state, initial_time=self._current_time,
+                       stop_at_time=stop_at_time, stop_at_steady=stop_at_steady, stop_criteria=stop_criteria,
+                       minimum_time_step_count=minimum_time_step_count, post_step_callback=post_step,
+                       method=stepper, step_size=control, linear_setup=self._setup_block_thomas,
+                       linear_solve=self._solve_block_thomas, linear_setup_rate=maximum_steps_per_jacobian,
+                       norm_weighting=w, save_each_step=not save_first_and_last_only, return_info=True)
+        if save_first_and_last_only:
+            state, t, _, info = out
+            ts, states = np.array([t]), state[None, :]
+        else:
+            ts, states, info = out
+        self._current_state = np.copy(states[-1])
+        self._current_time = float(ts[-1])
+        self.info = info
+        hist = FlameletHistory(ts, self._z)
+        ne = self._n_equations
+        names = self._mechanism.species_names
+        T = np.empty((ts.size, self._z.size))
+        T[:, 0], T[:, -1] = self._oxy_stream.T, self._fuel_stream.T
+        T[:, 1:-1] = states[:, ::ne]
+        hist['temperature'] = T
+        hist['pressure'] = np.full_like(T, self._pressure)
+        last = np.ones_like(T)
+        for i, sp in enumerate(names[:-1]):
+            Y = np.empty_like(T)
+            Y[:, 0], Y[:, -1] = self._oxy_stream.Y[i], self._fuel_stream.Y[i]
+            Y[:, 1:-1] = states[:, 1 + i::ne]
+            hist['mass fraction ' + sp] = Y
+            last -= Y
+        hist['mass fraction ' + names[-1]] = last
+        return (hist, info) if return_info else hist
+
+    def integrate_to_steady(self, steady_tolerance=1.e-4, **kwargs):
+        return self.integrate(stop_at_steady=steady_tolerance, **kwargs)
+
+    def integrate_to_time(self, final_time, **kwargs):
+        return self.integrate(stop_at_time=final_time, **kwargs)
 
     # ---------------- accessors ----------------
     @property
@@ -382,6 +468,10 @@ class Flamelet:
     @property
     def current_interior_state(self):
         return self._current_state.copy()
+
+    @property
+    def current_time(self):
+        return self._current_time
 
     @property
     def iteration_count(self):

@@ -137,6 +137,65 @@ def run_flamelets(spec, tol=1e-8, cpu=True, library_restarts=1, library_size=512
     return out
 
 
+def adjoint_reactors(spec, n, seed=0):
+    """n H2/air reactors at 1 atm: T0 ~ U[1000, 1600] K, equivalence ratio ~ U[0.5, 2]"""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    T0 = rng.uniform(1000., 1600., n)
+    phi = rng.uniform(0.5, 2.0, n)
+    names = list(spec.species_names)
+    X = np.zeros((n, spec.n_species))
+    X[:, names.index('H2')], X[:, names.index('O2')], X[:, names.index('N2')] = 2. * phi, 1., 3.76
+    Y = X * spec.molecular_weights[None, :]
+    return T0, Y / Y.sum(axis=1, keepdims=True)
+
+
+def run_adjoint(spec, n=1024, t_end=2.e-3, cpu=True):
+    import numpy as np
+    import torch
+    from spitfire_amd.reactor_batch import ReactorBatch
+    T0, Y0 = adjoint_reactors(spec, n)
+    print('adjoint: %d reactors to %g s' % (n, t_end), file=sys.stderr, flush=True)
+    rb = ReactorBatch(spec, T0, Y0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rb.integrate(t_end, log_rate=100)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L, G = rb.heat_release_gradient(log_rate=200)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    G = G.cpu().numpy()
+    if not (np.all(np.isfinite(G)) and bool(torch.isfinite(L).all())):
+        raise RuntimeError('non-finite adjoint gradient')
+    out = {'workload': 'gradient of int qdot dt over [0, %g s] w.r.t. ln A of all %d reactions, %d closed adiabatic '
+                       'isobaric H2/air reactors (T0 ~ U[1000,1600] K, phi ~ U[0.5,2], 1 atm, seed 0), ESDIRK64 '
+                       'forward with PI control (tolerance 1e-10) and its discrete adjoint' % (t_end, spec.n_reactions, n),
+           'reactors': n, 'seconds_forward': t1 - t0, 'seconds_adjoint': t2 - t1,
+           'gradients_per_s': n / (t2 - t0), 'steps': rb.stats['steps'],
+           'newton_iterations': rb.stats['newton_iterations'],
+           'final_T_range': [float(rb.final_state[:, 0].min()), float(rb.final_state[:, 0].max())],
+           'mean_heat_release_J_per_m3': float(L.mean())}
+    print('adjoint: forward %.2f s, adjoint %.2f s' % (t1 - t0, t2 - t1), file=sys.stderr, flush=True)
+    del rb
+    torch.cuda.empty_cache()
+    if cpu:
+        # what the reference offers for this gradient: central differences of its reactor integration,
+        # 2 nr + 1 integrations per gradient; one integration of the first reactor timed on the oracle
+        from oracle.reactor_oracle import OracleReactor
+        from spitfire_amd.mechanism import Stream
+        o = OracleReactor(spec, Stream(spec, T0[0], 101325., Y0[0]))
+        c0 = time.perf_counter()
+        o.integrate_to_time(t_end)
+        dt = time.perf_counter() - c0
+        out['cpu_baseline'] = {'value': 1. / (dt * (2 * spec.n_reactions + 1)), 'unit': 'gradients/s', 'cores': 1,
+                               'kind': 'port',
+                               'sample': 'one reactor (the first of the batch) integrated to %g s by the reference '
+                                         'step loop on the oracle kernels in %.2f s; a gradient by central differences '
+                                         'needs %d such integrations' % (t_end, dt, 2 * spec.n_reactions + 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -147,6 +206,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-flamelets', action='store_true')
     ap.add_argument('--library-size', type=int, default=512, help='flamelets in the library run')
+    ap.add_argument('--no-adjoint', action='store_true')
+    ap.add_argument('--adjoint-reactors', type=int, default=1024)
     ap.add_argument('--library-restarts', type=int, default=1,
                     help='pseudo-transient restarts allowed per flamelet in the 512-flamelet library run')
     ap.add_argument('--copy-calibration-bytes', type=int, default=0,
@@ -232,6 +293,8 @@ def main():
         torch.cuda.empty_cache()
         out['flamelets'] = run_flamelets(spec, cpu=not args.no_cpu_baseline, library_restarts=args.library_restarts,
                                        library_size=args.library_size)
+    if not args.no_adjoint:
+        out['adjoint'] = run_adjoint(spec, n=args.adjoint_reactors, cpu=not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
 
 

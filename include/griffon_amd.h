@@ -112,6 +112,16 @@ int gf_batch_reactor_rhs_isobaric(gf_mech *m, int n, const double *x, long x_ite
 int gf_batch_reactor_jac_isobaric(gf_mech *m, int n, const double *x, long x_item_stride, long x_var_stride,
                                   const double *p, long p_stride, double *rhs, long rhs_item_stride,
                                   long rhs_var_stride, double *jac, long jac_item_stride, void *stream);
+/* The same evaluation with the extra per-state terms a discrete adjoint of reactor integrations needs (no
+ * counterpart in the reference; it serves gradients of integrated heat release with respect to the rate
+ * constants): aux[4 i..] = {rho, cp, dcp/dT, mixture molecular weight}, cpi[ns i..] species heat capacities,
+ * h[ns i..] species enthalpies per unit mass, qa[nr i..] = d(rate of progress of reaction r)/d(ln A_r), reactions
+ * in griffon order (type-sorted, mechanism.py:540), A the forward (high-pressure, for falloff) pre-exponential
+ * factor. rhs and every extra output may be null. */
+int gf_batch_reactor_jac_isobaric_terms(gf_mech *m, int n, const double *x, long x_item_stride, long x_var_stride,
+                                        const double *p, long p_stride, double *rhs, long rhs_item_stride,
+                                        long rhs_var_stride, double *jac, long jac_item_stride, double *aux,
+                                        double *cpi, double *h, double *qa, void *stream);
 
 /* Batched block-tridiagonal (BTDDOD) factor/solve over nsys independent systems of possibly different
  * block counts: system s has nblocks[s] blocks of size bs; its values start at values + voff[s] (griffon

@@ -1,0 +1,121 @@
+"""TEST INFRASTRUCTURE ONLY — spitfire_amd.reactor_batch.ReactorBatch with its chemistry and linear algebra on
+the CPU oracle (oracle/griffon_oracle.cpp, SciPy LU) instead of the GPU: the checker for the batched reactor
+integration and its discrete adjoint. The step loop and the adjoint recursion are the product's, unchanged.
+
+The oracle has no per-reaction rate-of-progress output, so d(q_r)/d(ln A_r) is taken here from central
+differences of the oracle's production rates over mechanisms with A_r scaled by exp(+-eps):
+d w_k / d ln A_r = M_k nu_kr dq_r/dln A_r for any species k in reaction r."""
+import copy
+
+import numpy as np
+from scipy.linalg import lu_factor, lu_solve
+
+from oracle.oracle import CpuGriffon
+from spitfire_amd.mechdata import CompiledMechanism
+from spitfire_amd.reactor_batch import ReactorBatch
+
+
+def scaled_mechanism(mech_data, file_index, factor):
+    """mech_data with the forward (high-pressure, for falloff) pre-exponential of one reaction scaled"""
+    d = copy.deepcopy(mech_data)
+    rx = d['reactions'][file_index]
+    key = 'A' if 'A' in rx else 'fwd-A'
+    rx[key] = rx[key] * factor
+    return d
+
+
+class _CpuLU:
+    def __init__(self, R, ns):
+        self.R, self.ns = R, ns
+
+    def factor(self, values, stream):
+        V = values.numpy().reshape(self.R, self.ns, self.ns)  # [reactor, column, row]
+        self.f = [lu_factor(V[r].T) for r in range(self.R)]
+
+    def solve(self, rhs, out, stream):
+        import torch
+        r = rhs.numpy()
+        out.copy_(torch.from_numpy(np.stack([lu_solve(self.f[i], r[i]) for i in range(self.R)])))
+        return out
+
+
+class OracleReactorBatch(ReactorBatch):
+    _device = 'cpu'
+
+    def __init__(self, mech_spec, T0, Y0, pressure=101325., qa_eps=1.e-5):
+        self._cpu = CpuGriffon(CompiledMechanism(mech_spec.mech_data, order='name'))
+        self._qa_eps = qa_eps
+        self._perturbed = None
+        self._pressure = float(pressure)
+        super().__init__(mech_spec, T0, Y0, pressure)
+
+    def _make_lu(self):
+        return _CpuLU(self.R, self.ns)
+
+    def _soa(self, x):
+        return np.ascontiguousarray(x.numpy().T)
+
+    def _rhs(self, x, out):
+        import torch
+        out.copy_(torch.from_numpy(self._cpu.batch_rhs(self._soa(x), np.full(self.R, self._pressure))))
+        return out
+
+    def _jac(self, x, rhs, jac):
+        import torch
+        r, j = self._cpu.batch_rhs_jac(self._soa(x), np.full(self.R, self._pressure))
+        rhs.copy_(torch.from_numpy(r))
+        jac.copy_(torch.from_numpy(j))
+
+    def _full_y(self, xr):
+        return np.hstack([xr[1:], 1. - np.sum(xr[1:])])
+
+    def _terms(self, x, rhs, jac, aux, cpi, h, qa):
+        import torch
+        self._jac(x, rhs, jac)
+        X = x.numpy()
+        Ru = self._mech.mech_data['gas_constant']
+        A, C, H = np.zeros((self.R, 4)), np.zeros((self.R, self.ns)), np.zeros((self.R, self.ns))
+        for r in range(self.R):
+            y = self._full_y(X[r])
+            sc, cp_i, h_i, _ = self._cpu.thermo(X[r, 0], y)
+            A[r] = [self._pressure * sc[0] / (Ru * X[r, 0]), sc[1], sc[2], sc[0]]
+            C[r], H[r] = cp_i, h_i
+        aux.copy_(torch.from_numpy(A))
+        cpi.copy_(torch.from_numpy(C))
+        h.copy_(torch.from_numpy(H))
+        if qa is not None:
+            qa.copy_(torch.from_numpy(self._qa(X, A[:, 0])))
+
+    def _qa(self, X, rho):
+        if self._perturbed is None:
+            md = self._mech.mech_data
+            e = self._qa_eps
+            self._perturbed = [
+                (CpuGriffon(CompiledMechanism(scaled_mechanism(md, int(fi), np.exp(e)), order='name')),
+                 CpuGriffon(CompiledMechanism(scaled_mechanism(md, int(fi), np.exp(-e)), order='name')))
+                for fi in self.file_order]
+        N = self.N.numpy()
+        mw = self.mw.numpy()
+        out = np.zeros((self.R, self.nr))
+        for r in range(self.R):
+            y = self._full_y(X[r])
+            for g, (plus, minus) in enumerate(self._perturbed):
+                d = (plus.production_rates(X[r, 0], rho[r], y) - minus.production_rates(X[r, 0], rho[r], y)) / (
+                    2. * self._qa_eps)
+                k = int(np.argmax(np.abs(N[:, g])))
+                out[r, g] = d[k] / (mw[k] * N[k, g])
+        return out
+
+    def heat_release(self):
+        import torch
+        f = torch.empty((self.R, self.ns), dtype=torch.float64)
+        J = torch.empty((self.R, self.ns * self.ns), dtype=torch.float64)
+        aux = torch.empty((self.R, 4), dtype=torch.float64)
+        cpi = torch.empty((self.R, self.ns), dtype=torch.float64)
+        hs = torch.empty((self.R, self.ns), dtype=torch.float64)
+        L = torch.zeros(self.R, dtype=torch.float64)
+        for h, Xs, clip in self.records:
+            for i in range(6):
+                self._terms(Xs[i].contiguous(), f, J, aux, cpi, hs, None)
+                L += h * self.b[i] * aux[:, 0] * aux[:, 1] * f[:, 0]
+        return L

@@ -400,6 +400,18 @@ int gf_batch_reactor_jac_isobaric(gf_mech *m, int n, const double *x, long xs_i,
   return launch_chem(m, io, static_cast<hipStream_t>(stream));
 }
 
+int gf_batch_reactor_jac_isobaric_terms(gf_mech *m, int n, const double *x, long xs_i, long xs_v, const double *p,
+                                        long ps, double *rhs, long rs_i, long rs_v, double *jac, long js_i, double *aux,
+                                        double *cpi, double *h, double *qa, void *stream) {
+  if (!m) return fail(GF_ERR_ARG, "null mechanism");
+  if (n < 0 || !x || !p || !jac) return fail(GF_ERR_ARG, "bad batch arguments");
+  GF_TRY(ensure_device(m));
+  ChemIO io;
+  io.n = n; io.x = x; io.xs_i = xs_i; io.xs_v = xs_v; io.p = p; io.ps = ps; io.rhs = rhs; io.rs_i = rs_i; io.rs_v = rs_v;
+  io.jac = jac; io.js_i = js_i; io.aux = aux; io.cpi = cpi; io.hsp = h; io.qa = qa;
+  return launch_chem(m, io, static_cast<hipStream_t>(stream));
+}
+
 int gf_batch_btddod_full_factorize(int nsys, const int *nblocks, int bs, double *values, const long *voff,
                                    double *l_values, const long *loff, int *pivots, const long *poff, void *stream) {
   return launch_btd_factor(nsys, nblocks, bs, values, voff, l_values, loff, pivots, poff,

@@ -199,6 +199,7 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
         const double cpsm = wave_sum((lane < ns && lane > mv.last_const) ? yk * cps : 0.);
         if (a.aux && lane < 4) a.aux[i * 4 + lane] = lane == 0 ? rho : (lane == 1 ? cpm : (lane == 2 ? cpsm : mmw));
         if (a.cpi && lane < ns) a.cpi[i * ns + lane] = cp;
+        if (a.hsp && lane < ns) a.hsp[i * ns + lane] = h;
         if (lane == 0) {
           sc[S_T] = T; sc[S_RHO] = rho; sc[S_MMW] = mmw; sc[S_CP] = cpm; sc[S_CPST] = cpsm;
           sc[S_INVT] = invT; sc[S_LOGT] = logT; sc[S_CT] = rho * mm; sc[S_INVM] = mm;
@@ -269,6 +270,7 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
       }
 
       double Ctbaf = 1., dCdrho = 0., dCdT = 0., cdense = 0., tbs = 0.;
+      double qaC = 0.;  // d(q)/d(ln A) = Rnet * qaC; A is the forward (high-pressure) pre-exponential
       if (R.type == RK_THIRD_BODY) {
         Ctbaf = R.baseEff * ct + rho * tbsum;
         dCdrho = R.baseEff * invM + tbsum;
@@ -305,6 +307,8 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
         dCdrho = nsT * R.baseEff * invM + nsT * tbsum;
         cdense = nsT * rho * R.baseEff;
         tbs = nsT * rho;
+        // Pr = k0 [M] / kinf: d ln(q) / d ln(kinf) = 1 - (Pr / Ctbaf) dCtbaf/dPr, and nsT = (k0 / kinf) dCtbaf/dPr
+        qaC = Ctbaf - flf * nsT;
       }
 
       double *vr = v + R.slot_off;
@@ -390,6 +394,10 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
         else if (q == R.tb_last) Elast = tbs * e;
       }
       rq[r] = Rnet * Ctbaf;
+      if (a.qa) {
+        const long q = a.sel ? (long)a.sel[i] : i;
+        a.qa[q * nr + r] = Rnet * ((R.type == RK_LINDEMANN || R.type == RK_TROE) ? qaC : Ctbaf);
+      }
       rq[nr + r] = dRdrho * Ctbaf + dCdrho * Rnet;
       rq[2 * nr + r] = dRdT * Ctbaf + dCdT * Rnet;
       rq[3 * nr + r] = Rnet * cdense;

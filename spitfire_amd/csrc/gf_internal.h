@@ -141,6 +141,8 @@ struct ChemIO {
   double *sens = nullptr;  // raw option-0 sensitivities [n][(ns+1)^2]
   double *aux = nullptr;   // per state {rho, cp, dcp/dT, mixture molecular weight}
   double *cpi = nullptr;   // per state species heat capacities [n][ns]
+  double *hsp = nullptr;   // per state species enthalpies per unit mass [n][ns]
+  double *qa = nullptr;    // per state d(rate of progress)/d(ln A) of every reaction [n][nr] (Jacobian kernel)
   // optional: logical state i reads and writes physical row sel[i] of every array above
   const int *sel = nullptr;
   // optional: the Jacobian block of physical row q starts at jac + js_off[q] (no block if negative)

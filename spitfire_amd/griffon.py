@@ -302,6 +302,21 @@ class PyCombustionKernels:
                                                       jac.data_ptr(), self._ns * self._ns, _stream(stream)))
 
 
+    def batch_reactor_jac_isobaric_terms(self, x, p, rhs, jac, aux, cpi, h, qa, stream=None):
+        """As batch_reactor_jac_isobaric with [n, ns] ('aos') states, plus per state aux [n, 4] = {rho, cp,
+        dcp/dT, mixture molecular weight}, cpi and h [n, ns] (species heat capacities and enthalpies per unit
+        mass) and qa [n, nr] = d(rate of progress)/d(ln A) per reaction in griffon order."""
+        n, xs = _strides(x, self._ns, 'aos')
+        for t, w in ((aux, 4), (cpi, self._ns), (h, self._ns), (qa, None)):
+            if t.dim() != 2 or t.shape[0] != n or (w is not None and t.shape[1] != w) or not t.is_contiguous():
+                raise ValueError('per-state outputs must be contiguous [n, k] tensors')
+        check(self._lib.gf_batch_reactor_jac_isobaric_terms(self._h, n, x.data_ptr(), xs[0], xs[1], p.data_ptr(),
+                                                            1 if p.numel() > 1 else 0, rhs.data_ptr(), self._ns, 1,
+                                                            jac.data_ptr(), self._ns * self._ns, aux.data_ptr(),
+                                                            cpi.data_ptr(), h.data_ptr(), qa.data_ptr(),
+                                                            _stream(stream)))
+
+
 def _strides(t, ns, layout):
     import torch
     if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or t.dim() != 2:

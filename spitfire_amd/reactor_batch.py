@@ -1,0 +1,296 @@
+"""Many closed adiabatic isobaric reactors integrated together on the GPU, and the discrete adjoint of the
+integrated heat release with respect to the pre-exponential factor of every reaction.
+
+Forward: the step loop of HomogeneousReactor.integrate with its defaults (reactors.py:429-678: ESDIRK64,
+methods.py:441-612; frozen-Jacobian Newton, nonlinear.py:185-268, tolerance 1e-12 on the scaled residual;
+PI control, stepcontrol.py:50-117; a Jacobian setup every step; negative entries clipped after every step),
+run for all reactors in lock step. Each reactor keeps its own time, step size, Newton convergence and clipping.
+The chemistry (RHS, Jacobian) runs in the batched HIP kernel; every dt*gamma*J - I is factored and solved by the
+batched block LU with one block per system.
+
+Loss: L = integral over [0, t_end] of qdot = -sum_k h_k w_k (= rho cp dT/dt for a closed adiabatic isobaric
+reactor, isobaric_reactor_kernels.cpp:23), discretised by the Runge-Kutta weights of each step, i.e. as an
+extra state Q' = qdot integrated by the same method.
+
+Adjoint: with the accepted step sizes frozen and the stage equations taken as solved exactly, the discrete
+adjoint of one step k_i = f(X_i), X_i = x_n + dt sum_j a_ij k_j, x_{n+1} = x_n + dt sum_i b_i k_i is
+(Hager, Numer. Math. 87, 2000; Sandu, 2006)
+    theta_i = dt J_i^T w_i + dt b_i grad(qdot)_i,   w_i = b_i lambda_{n+1} + sum_j a_ji theta_j,
+    lambda_n = lambda_{n+1} + sum_i theta_i,
+    dL/dp += dt sum_i [ (df/dp)_i^T w_i + b_i (dqdot/dp)_i ],
+solved stage by stage from the last (the tableau is lower triangular, so each implicit stage needs one solve
+with I - dt a_ii J_i^T). A clipped component has zero derivative through the clip. For p = ln A_r,
+df/dp = (df/dq_r) dq_r/dln A_r, with dq_r/dln A_r from the chemistry kernel (the rate of progress itself for
+elementary and three-body reactions; for falloff reactions A is the high-pressure factor and the falloff
+function's dependence on it is included).
+"""
+import numpy as np
+
+from .time_integration import KennedyCarpenterS6P4Q3, SimpleNewtonSolver
+
+
+class _BatchedLU:
+    """Dense LU with partial pivoting of R independent ns x ns column-major blocks (the block-tridiagonal
+    kernels with one block per system)."""
+
+    def __init__(self, R, ns, device):
+        import torch
+        from . import _native
+        self._lib = _native.lib()
+        self._check = _native.check
+        self.R, self.ns = R, ns
+        i64 = dict(dtype=torch.int64, device=device)
+        self.nb = torch.ones(R, dtype=torch.int32, device=device)
+        self.voff = torch.arange(R, **i64) * ns * ns
+        self.poff = torch.arange(R, **i64) * ns
+        self.L = torch.zeros(R * ns * ns, dtype=torch.float64, device=device)
+        self.piv = torch.zeros(R * ns, dtype=torch.int32, device=device)
+        self.values = None
+
+    def factor(self, values, stream):
+        """values: contiguous [R, ns*ns] column-major blocks, factored in place"""
+        self.values = values
+        self._check(self._lib.gf_batch_btddod_full_factorize(self.R, self.nb.data_ptr(), self.ns, values.data_ptr(),
+                                                             self.voff.data_ptr(), self.L.data_ptr(),
+                                                             self.voff.data_ptr(), self.piv.data_ptr(),
+                                                             self.poff.data_ptr(), stream))
+
+    def solve(self, rhs, out, stream):
+        self._check(self._lib.gf_batch_btddod_full_solve(self.R, self.nb.data_ptr(), self.ns, self.values.data_ptr(),
+                                                         self.voff.data_ptr(), self.L.data_ptr(), self.voff.data_ptr(),
+                                                         self.piv.data_ptr(), self.poff.data_ptr(), rhs.data_ptr(),
+                                                         out.data_ptr(), self.poff.data_ptr(), stream))
+        return out
+
+
+class ReactorBatch:
+    """R closed adiabatic isobaric reactors of one mechanism at one pressure, on cuda:0.
+
+    T0: [R] initial temperatures; Y0: [R, ns] initial mass fractions; pressure in Pa."""
+    _device = 'cuda:0'
+
+    def __init__(self, mech_spec, T0, Y0, pressure=101325.):
+        import torch
+        self._mech = mech_spec
+        self._g = mech_spec.griffon
+        self.dev = torch.device(self._device)
+        ns = mech_spec.n_species
+        self.ns, self.nr = ns, mech_spec.n_reactions
+        T0 = np.asarray(T0, dtype=np.float64)
+        Y0 = np.asarray(Y0, dtype=np.float64)
+        self.R = T0.size
+        x0 = np.hstack([T0[:, None], Y0[:, :-1]])
+        self.x0 = torch.tensor(x0, device=self.dev)
+        self.p = torch.tensor([float(pressure)], dtype=torch.float64, device=self.dev)
+        self.w = torch.ones(ns, dtype=torch.float64, device=self.dev)
+        self.w[0] = 1.e-3  # norm weighting 1/variable scale (reactors.py:311-312)
+        m = KennedyCarpenterS6P4Q3(SimpleNewtonSolver())
+        self.A, self.b, self.bh, self.gamma = m.A, m.b, m.bh, m.gamma
+        # net stoichiometry (products - reactants) in griffon reaction order, molecular weights
+        names = list(mech_spec.species_names)
+        idx = {n: i for i, n in enumerate(names)}
+        N = np.zeros((ns, self.nr))
+        for g, rx in enumerate(mech_spec._reactions):
+            for s, c in rx['reactants'].items():
+                N[idx[s], g] -= c
+            for s, c in rx['products'].items():
+                N[idx[s], g] += c
+        self.N = torch.tensor(N, device=self.dev)
+        self.mw = torch.tensor(mech_spec.molecular_weights, device=self.dev)
+        self.file_order = np.asarray(mech_spec._reaction_file_order)
+        self.lu = self._make_lu()
+        self.records = None
+
+    def _make_lu(self):
+        return _BatchedLU(self.R, self.ns, self.dev)
+
+    # ---------------- kernels ----------------
+    def _stream(self):
+        import torch
+        return torch.cuda.current_stream().cuda_stream if self.dev.type == 'cuda' else None
+
+    def _rhs(self, x, out):
+        self._g.batch_reactor_rhs_isobaric(x, self.p, out, layout='aos')
+        return out
+
+    def _jac(self, x, rhs, jac):
+        self._g.batch_reactor_jac_isobaric(x, self.p, rhs, jac, layout='aos')
+
+    # ---------------- forward ----------------
+    def integrate(self, t_end, first_time_step=1.e-6, max_time_step=1.e6, transient_tolerance=1.e-10,
+                  nonlinear_solve_tolerance=1.e-12, max_nonlinear_iter=20, dt_schedule=None, record=True,
+                  max_steps=100000, log_rate=0):
+        """Integrate every reactor from its initial state to t_end. With dt_schedule (a [steps, R] array of
+        step sizes, 0 for no step) the step sizes are replayed instead of controlled (the frozen-step map that
+        the adjoint differentiates). Returns the final states [R, ns]."""
+        import torch
+        R, ns = self.R, self.ns
+        A, b, bh, g = self.A, self.b, self.bh, self.gamma
+        w = self.w
+        x = self.x0.clone()
+        t = torch.zeros(R, dtype=torch.float64, device=self.dev)
+        dt = torch.full((R,), float(first_time_step), dtype=torch.float64, device=self.dev)
+        f = torch.empty_like(x)
+        J = torch.empty((R, ns * ns), dtype=torch.float64, device=self.dev)
+        eye = torch.eye(ns, dtype=torch.float64, device=self.dev)
+        d = torch.empty_like(x)
+        st = self._stream()
+        recs = []
+        nsteps = 0
+        self.stats = {'steps': 0, 'newton_iterations': 0, 'max_newton_iterations_hit': 0}
+        while True:
+            if dt_schedule is not None:
+                if nsteps >= len(dt_schedule):
+                    break
+                h = torch.tensor(np.asarray(dt_schedule[nsteps], dtype=np.float64), device=self.dev)
+            else:
+                active = t < t_end
+                if not bool(active.any()) or nsteps >= max_steps:
+                    break
+                dt = torch.where(t + dt > t_end, t_end - t, dt)
+                h = torch.where(active, dt, torch.zeros_like(dt))
+            xn = x.clone()
+            self._jac(xn, f, J)
+            M = (h[:, None, None] * g * J.view(R, ns, ns) - eye).reshape(R, ns * ns).contiguous()
+            self.lu.factor(M, st)
+            ks = [f.clone()]
+            X = [xn.clone()]
+            xs = xn.clone()
+            for s in range(1, 6):
+                prior = sum(A[s, j] * ks[j] for j in range(s))
+                xi = xs.clone()
+                fi = ks[-1].clone()
+                r = h[:, None] * (g * fi + prior) - (xi - xn)
+                conv = torch.zeros(R, dtype=torch.bool, device=self.dev)
+                for it in range(max_nonlinear_iter):
+                    self.lu.solve(r, d, st)
+                    xi = torch.where(conv[:, None], xi, xi - d)
+                    self._rhs(xi, fi)
+                    r = h[:, None] * (g * fi + prior) - (xi - xn)
+                    conv = conv | (torch.max(torch.abs(r * w), dim=1).values < nonlinear_solve_tolerance)
+                    self.stats['newton_iterations'] += 1
+                    if bool(conv.all()):
+                        break
+                else:
+                    self.stats['max_newton_iterations_hit'] += 1
+                ks.append(fi.clone())
+                X.append(xi.clone())
+                xs = xi
+            dx = h[:, None] * sum(b[j] * ks[j] for j in range(6))
+            err = torch.max(torch.abs(h[:, None] * sum((b[j] - bh[j]) * ks[j] for j in range(6)) * w), dim=1).values
+            ok = torch.isfinite(dx).all(dim=1)
+            hacc = torch.where(ok, h, torch.zeros_like(h))
+            xnew = xn + torch.where(ok[:, None], dx, torch.zeros_like(dx))
+            clip = xnew < 0.
+            x = torch.where(clip, torch.zeros_like(xnew), xnew)
+            t = t + hacc
+            if dt_schedule is None:
+                # PI control (stepcontrol.py:84-101; the step count is at least 1 here, so the history ratio is 1)
+                ramp = torch.clamp((transient_tolerance / err) ** 0.1333333333, max=1.1)
+                dnew = torch.where(err < 1.e-16, dt * 1.1, dt * ramp)
+                dnew = torch.clamp(dnew, max=max_time_step)
+                dt = torch.where(ok, dnew, dt * 0.8)  # non-finite update: reject, shrink (integrator.py)
+            if record:
+                Xs = torch.stack([torch.where(ok[:, None], Xi, xn) for Xi in X])
+                recs.append((hacc.clone(), Xs, clip))
+            nsteps += 1
+            if log_rate and nsteps % log_rate == 0:
+                import sys
+                print('reactor batch step %d: min t %.3e' % (nsteps, float(t.min())), file=sys.stderr, flush=True)
+        self.stats['steps'] = nsteps
+        self.records = recs if record else None
+        self.final_state = x
+        self.final_time = t
+        return x
+
+    def step_sizes(self):
+        """[steps, R] accepted step sizes of the last recorded integration (0: no step)"""
+        return np.stack([r[0].cpu().numpy() for r in self.records])
+
+    # ---------------- adjoint ----------------
+    def _terms(self, x, rhs, jac, aux, cpi, h, qa):
+        self._g.batch_reactor_jac_isobaric_terms(x, self.p, rhs, jac, aux, cpi, h, qa)
+
+    def heat_release(self):
+        """L [R] of the recorded integration (the same quadrature the adjoint differentiates)"""
+        import torch
+        R, ns, nr, dev = self.R, self.ns, self.nr, self.dev
+        f = torch.empty((R, ns), dtype=torch.float64, device=dev)
+        J = torch.empty((R, ns * ns), dtype=torch.float64, device=dev)
+        aux = torch.empty((R, 4), dtype=torch.float64, device=dev)
+        cpi = torch.empty((R, ns), dtype=torch.float64, device=dev)
+        hs = torch.empty((R, ns), dtype=torch.float64, device=dev)
+        qa = torch.empty((R, nr), dtype=torch.float64, device=dev)
+        L = torch.zeros(R, dtype=torch.float64, device=dev)
+        for h, Xs, clip in self.records:
+            for i in range(6):
+                self._terms(Xs[i].contiguous(), f, J, aux, cpi, hs, qa)
+                L += h * self.b[i] * aux[:, 0] * aux[:, 1] * f[:, 0]
+        return L
+
+    def heat_release_gradient(self, log_rate=0):
+        """Discrete adjoint over the recorded integration. Returns (L [R], dL/dlnA [R, nr]) with reactions in
+        the mechanism's declared (file) order."""
+        import torch
+        if not self.records:
+            raise RuntimeError('integrate(record=True) first')
+        R, ns, nr = self.R, self.ns, self.nr
+        A, b = self.A, self.b
+        dev = self.dev
+        f = torch.empty((R, ns), dtype=torch.float64, device=dev)
+        J = torch.empty((R, ns * ns), dtype=torch.float64, device=dev)
+        aux = torch.empty((R, 4), dtype=torch.float64, device=dev)
+        cpi = torch.empty((R, ns), dtype=torch.float64, device=dev)
+        hs = torch.empty((R, ns), dtype=torch.float64, device=dev)
+        qa = torch.empty((R, nr), dtype=torch.float64, device=dev)
+        eye = torch.eye(ns, dtype=torch.float64, device=dev)
+        dinvm = (1. / self.mw[:-1] - 1. / self.mw[-1])
+        lam = torch.zeros((R, ns), dtype=torch.float64, device=dev)
+        grad = torch.zeros((R, nr), dtype=torch.float64, device=dev)
+        L = torch.zeros(R, dtype=torch.float64, device=dev)
+        st = self._stream()
+        th = torch.empty((R, ns), dtype=torch.float64, device=dev)
+        for n_back, (h, Xs, clip) in enumerate(reversed(self.records)):
+            if log_rate and n_back % log_rate == 0:
+                import sys
+                print('adjoint step %d of %d' % (n_back, len(self.records)), file=sys.stderr, flush=True)
+            lam_p = torch.where(clip, torch.zeros_like(lam), lam)
+            thetas = [None] * 6
+            for i in range(5, -1, -1):
+                x = Xs[i].contiguous()
+                self._terms(x, f, J, aux, cpi, hs, qa)
+                rho, cp, dcp, mmw = aux[:, 0], aux[:, 1], aux[:, 2], aux[:, 3]
+                Jc = J.view(R, ns, ns)  # Jc[r, column, row]
+                fT = f[:, 0]
+                # gradient of qdot = rho cp dT/dt with respect to (T, Y_0 .. Y_{ns-2})
+                drho = torch.empty((R, ns), dtype=torch.float64, device=dev)
+                drho[:, 0] = -rho / x[:, 0]
+                drho[:, 1:] = -(rho * mmw)[:, None] * dinvm[None, :]
+                dcpv = torch.empty((R, ns), dtype=torch.float64, device=dev)
+                dcpv[:, 0] = dcp
+                dcpv[:, 1:] = cpi[:, :-1] - cpi[:, -1:]
+                gq = (rho * cp)[:, None] * Jc[:, :, 0] + fT[:, None] * (cp[:, None] * drho + rho[:, None] * dcpv)
+                v = b[i] * lam_p
+                for j in range(i + 1, 6):
+                    if A[j, i] != 0.:
+                        v = v + A[j, i] * thetas[j]
+                rhs = h[:, None] * (torch.bmm(Jc, v[:, :, None])[:, :, 0] + b[i] * gq)
+                if i > 0:
+                    Mt = (eye - (h * A[i, i])[:, None, None] * Jc.transpose(1, 2)).reshape(R, ns * ns).contiguous()
+                    self.lu.factor(Mt, st)
+                    theta = self.lu.solve(rhs.contiguous(), th, st).clone()
+                else:
+                    theta = rhs
+                thetas[i] = theta
+                wv = v + A[i, i] * theta
+                wT = wv[:, 0] + b[i] * rho * cp
+                C = torch.zeros((R, ns), dtype=torch.float64, device=dev)
+                C[:, :-1] = wv[:, 1:] / rho[:, None]
+                C = C - (wT / (rho * cp))[:, None] * hs
+                grad += h[:, None] * qa * ((C * self.mw[None, :]) @ self.N)
+                L += h * b[i] * rho * cp * fT
+            lam = lam_p + sum(thetas)
+        out = torch.empty_like(grad)
+        out[:, torch.as_tensor(self.file_order, device=dev)] = grad
+        self.lambda0 = lam
+        return L, out

@@ -512,11 +512,27 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
               const double Mi = mv.mw[ii];
               const int ee = j * ns + ii;
               const int t0 = exo[ee], t1 = exo[ee + 1];
-              for (int t = t0; t < t1; ++t) {
+              const double *vb = sbase + NSA * SP_STRIDE + N_SCAL + 5 * nr;
+              int t = t0;
+              for (; t + 1 < t1; t += 2) {  // two entries per trip: their LDS loads overlap (same summation order)
+                const uint32_t ra = exx[t], rb = exx[t + 1];
+                const int sa = (int)(ra & 0xFFFFu), sb = (int)(rb & 0xFFFFu);
+                const double ca = -(double)(int8_t)(ra >> 16) * Mi, cb = -(double)(int8_t)(rb >> 16) * Mi;
+                double va[S], vbv[S];
+                for (int s = 0; s < S; ++s) {
+                  va[s] = vb[s * stride + sa];
+                  vbv[s] = vb[s * stride + sb];
+                }
+                for (int s = 0; s < S; ++s) {
+                  full[s] += ca * va[s];
+                  full[s] += cb * vbv[s];
+                }
+              }
+              if (t < t1) {
                 const uint32_t rec = exx[t];
                 const int slot = (int)(rec & 0xFFFFu);
                 const double coef = -(double)(int8_t)(rec >> 16) * Mi;
-                for (int s = 0; s < S; ++s) full[s] += coef * sbase[s * stride + NSA * SP_STRIDE + N_SCAL + 5 * nr + slot];
+                for (int s = 0; s < S; ++s) full[s] += coef * vb[s * stride + slot];
               }
               for (int s = 0; s < S; ++s) {
                 const double *sp = sbase + s * stride;

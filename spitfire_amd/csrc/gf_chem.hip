@@ -418,7 +418,29 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
         if (lane < ns) {
           const double Mk = mv.mw[lane];
           const uint32_t e0 = sgx_off[lane], e1 = sgx_off[lane + 1];
-          for (uint32_t e = e0; e < e1; ++e) {
+          uint32_t e = e0;
+          for (; e + 1 < e1; e += 2) {  // two reactions per trip: their LDS loads overlap (same summation order)
+            const uint64_t ra = sgx[e], rb = sgx[e + 1];
+            const int r0 = (int)(ra & 0xFFFFu), r1 = (int)(rb & 0xFFFFu);
+            const double c0 = -(double)(int8_t)(ra >> 32) * Mk, c1 = -(double)(int8_t)(rb >> 32) * Mk;
+            const double q0 = rq[r0], q1 = rq[r1];
+            if (JAC) {
+              const int d0 = (int)((ra >> 16) & 0xFFFFu), d1 = (int)((rb >> 16) & 0xFFFFu);
+              const double a00 = rq[nr + r0], a01 = rq[nr + r1], a10 = rq[2 * nr + r0], a11 = rq[2 * nr + r1];
+              const double a20 = rq[3 * nr + r0], a21 = rq[3 * nr + r1], a30 = rq[4 * nr + r0], a31 = rq[4 * nr + r1];
+              const double v0 = d0 != 0xFFFF ? v[d0] : 0., v1 = d1 != 0xFFFF ? v[d1] : 0.;
+              w += c0 * q0; w += c1 * q1;
+              w0 += c0 * a00; w0 += c1 * a01;
+              w1 += c0 * a10; w1 += c1 * a11;
+              A += c0 * a20; A += c1 * a21;
+              B += c0 * a30; B += c1 * a31;
+              if (d0 != 0xFFFF) DG += c0 * v0;
+              if (d1 != 0xFFFF) DG += c1 * v1;
+            } else {
+              w += c0 * q0; w += c1 * q1;
+            }
+          }
+          for (; e < e1; ++e) {
             const uint64_t rec = sgx[e];
             const int r = (int)(rec & 0xFFFFu);
             const double c = -(double)(int8_t)(rec >> 32) * Mk;  // -nu * M_k (rates_sensitivities_exact.cpp:1018)

@@ -106,12 +106,14 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld,
   while (nn >= 0) {
     int its = 0, l;
     for (;;) {
-      // find a negligible subdiagonal element
-      for (l = nn; l >= 1; --l) {
-        double s = fabs(a[(l - 1) * ld + l - 1]) + fabs(a[l * ld + l]);
+      // the negligible subdiagonal element closest to the bottom of the window (0 if none): the largest l in
+      // [1, nn] passing the test, evaluated for every l at once (the same l as a downward scan)
+      l = par.last_true(1, nn + 1, [&](int i) {
+        double s = fabs(a[(i - 1) * ld + i - 1]) + fabs(a[i * ld + i]);
         if (s == 0.) s = anorm;
-        if (fabs(a[l * ld + l - 1]) + s == s) break;
-      }
+        return fabs(a[i * ld + i - 1]) + s == s;
+      });
+      if (l < 1) l = 0;
       const double x = a[nn * ld + nn];
       if (l == nn) {  // one real root
         emax = fmax(emax, x + t);
@@ -152,24 +154,29 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld,
       }
       ++its;
       ++total;
-      // look for two consecutive small subdiagonal elements
-      int m;
-      double p = 0., q = 0., r = 0.;
-      for (m = nn - 2; m >= l; --m) {
-        const double z = a[m * ld + m];
+      // look for two consecutive small subdiagonal elements: the largest m in [l+1, nn-2] passing the test
+      // (evaluated for every m at once; the same m as a downward scan), else m = l; then the shift vector at m
+      auto shift_vector = [&](int mm, double &pp, double &qq, double &rr3) {
+        const double z = a[mm * ld + mm];
         const double rr = xx - z, ss = yy - z;
-        p = (rr * ss - ww) / a[(m + 1) * ld + m] + a[m * ld + m + 1];
-        q = a[(m + 1) * ld + m + 1] - z - rr - ss;
-        r = a[(m + 2) * ld + m + 1];
-        const double is = 1. / (fabs(p) + fabs(q) + fabs(r));
-        p *= is;
-        q *= is;
-        r *= is;
-        if (m == l) break;
-        const double u = fabs(a[m * ld + m - 1]) * (fabs(q) + fabs(r));
-        const double v = fabs(p) * (fabs(a[(m - 1) * ld + m - 1]) + fabs(z) + fabs(a[(m + 1) * ld + m + 1]));
-        if (u + v == v) break;
-      }
+        pp = (rr * ss - ww) / a[(mm + 1) * ld + mm] + a[mm * ld + mm + 1];
+        qq = a[(mm + 1) * ld + mm + 1] - z - rr - ss;
+        rr3 = a[(mm + 2) * ld + mm + 1];
+        const double is = 1. / (fabs(pp) + fabs(qq) + fabs(rr3));
+        pp *= is;
+        qq *= is;
+        rr3 *= is;
+      };
+      int m = par.last_true(l + 1, nn - 1, [&](int i) {
+        double pi, qi, ri;
+        shift_vector(i, pi, qi, ri);
+        const double u = fabs(a[i * ld + i - 1]) * (fabs(qi) + fabs(ri));
+        const double v = fabs(pi) * (fabs(a[(i - 1) * ld + i - 1]) + fabs(a[i * ld + i]) + fabs(a[(i + 1) * ld + i + 1]));
+        return u + v == v;
+      });
+      if (m < l + 1) m = l;
+      double p = 0., q = 0., r = 0.;
+      shift_vector(m, p, q, r);
       par.each(m + 2, nn + 1, [&](int i) {
         a[i * ld + i - 2] = 0.;
         if (i != m + 2) a[i * ld + i - 3] = 0.;
@@ -230,6 +237,12 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld,
 // Host policy: lane loops run sequentially (used by the CPU unit tests of this file).
 struct HostLanes {
   template <class F> void each(int lo, int hi, F f) { for (int t = lo; t < hi; ++t) f(t); }
+  // the largest i in [lo, hi) with pred(i), or lo - 1 if none
+  template <class F> int last_true(int lo, int hi, F pred) {
+    for (int t = hi - 1; t >= lo; --t)
+      if (pred(t)) return t;
+    return lo - 1;
+  }
   template <class F> double sum(int lo, int hi, F f) {
     double s = 0.;
     for (int t = lo; t < hi; ++t) s += f(t);

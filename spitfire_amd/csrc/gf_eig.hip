@@ -13,6 +13,12 @@ struct WaveLanes {
     if (lane >= lo && lane < hi) f(lane);
     __syncthreads();
   }
+  // the largest i in [lo, hi) with pred(i), or lo - 1 if none: every lane tests its own index, one ballot
+  template <class F> __device__ int last_true(int lo, int hi, F pred) {
+    const bool v = lane >= lo && lane < hi && pred(lane);
+    const unsigned long long mask = __ballot(v);
+    return mask ? 63 - __clzll((long long)mask) : lo - 1;
+  }
   template <class F> __device__ double sum(int lo, int hi, F f) {
     // DPP scan (no LDS permutes); the same value in every lane
     return wave::dpp_wave_sum((lane >= lo && lane < hi) ? f(lane) : 0.);

@@ -133,6 +133,13 @@ int gf_batch_btddod_full_solve(int nsys, const int *nblocks, int bs, const doubl
                                const double *l_values, const long *loff, const int *pivots, const long *poff,
                                const double *rhs, double *x, const long *xoff, void *stream);
 
+/* The pseudo-time limiter's eigenvalue bound for many independent blocks, as flamelet_jacobian computes it per
+ * point with compute_eigenvalues (flamelet_kernels.cpp:1329-1341): out[b] = max(max Re(lambda(B_b)) - diffterm, 0)
+ * for nblk column-major n x n blocks stored one after another (device pointers; n <= 64). nfail (a device int,
+ * may be null) is incremented for each block whose QR iteration did not converge. m may be null. */
+int gf_batch_eigenvalue_bound(gf_mech *m, int nblk, int n, const double *blocks, double diffterm, double *out,
+                              int *nfail, void *stream);
+
 /* The same systems factored for throughput (the form the batched flamelet solver uses): the block Thomas
  * recurrence of btddod_full_factorize (btddod_matrix_kernels.cpp:19-80) run from both ends of each chain at
  * once, storing the inverse of every Schur-updated diagonal block instead of LU factors and L blocks.

@@ -425,6 +425,14 @@ int gf_batch_btddod_full_solve(int nsys, const int *nblocks, int bs, const doubl
                           static_cast<hipStream_t>(stream));
 }
 
+int gf_batch_eigenvalue_bound(gf_mech *m, int nblk, int n, const double *blocks, double diffterm, double *out,
+                              int *nfail, void *stream) {
+  if (nblk < 0 || !blocks || !out) return fail(GF_ERR_ARG, "bad eigenvalue batch arguments");
+  if (m) GF_TRY(ensure_device(m));
+  return launch_expeig_list(m, blocks, nullptr, nullptr, nblk, n, diffterm, out, nfail, static_cast<hipStream_t>(stream),
+                            1);
+}
+
 int gf_batch_btddod_inverse_factorize(int nsys, const int *nblocks, int bs, const double *values, const long *voff,
                                       void *dinv, const long *doff, int precision, void *stream) {
   const hipStream_t st = static_cast<hipStream_t>(stream);

@@ -25,13 +25,14 @@ struct WaveLanes {
   }
 };
 
-// expeig for block b: max(0, max Re(lambda(B_b)) - diffterm), written to out[b*n .. b*n+n-1]
-// (flamelet_kernels.cpp:1329-1341). Blocks are column-major n x n at blocks + b*bstride.
-// With boff/ooff given, block b starts at blocks + boff[b] and its n outputs at out + ooff[b].
+// expeig for block b: max(0, max Re(lambda(B_b)) - diffterm), written nout times to out[b*nout ..]
+// (flamelet_kernels.cpp:1329-1341, which writes it once per point unknown: nout = n). Blocks are column-major
+// n x n at blocks + b*bstride. With boff/ooff given, block b starts at blocks + boff[b] and its outputs at
+// out + ooff[b].
 __global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ blocks, long bstride,
                                                     const long *__restrict__ boff, const long *__restrict__ ooff,
                                                     long nblk, int n, double diffterm, double *__restrict__ out,
-                                                    int *__restrict__ nfail) {
+                                                    int nout, int *__restrict__ nfail) {
   extern __shared__ double smem[];
   const int ld = n | 1;
   double *a = smem;
@@ -44,8 +45,8 @@ __global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ b
     int failed = 0;
     const double em = eig_max_real(par, a, vb, n, ld, &failed);
     const double ex = fmax(em - diffterm, 0.);
-    double *dst = out + (ooff ? ooff[b] : b * n);
-    for (int q = threadIdx.x; q < n; q += 64) dst[q] = ex;
+    double *dst = out + (ooff ? ooff[b] : b * nout);
+    for (int q = threadIdx.x; q < nout; q += 64) dst[q] = ex;
     if (failed && threadIdx.x == 0 && nfail) atomicAdd(nfail, 1);
     __syncthreads();
   }
@@ -54,16 +55,20 @@ __global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ b
 size_t expeig_lds_bytes(int n) { return ((size_t)n * (n | 1) + 64) * sizeof(double); }
 
 int launch_expeig_list(const gf_mech *m, const double *blocks, const long *boff, const long *ooff, long nblk, int n,
-                       double diffterm, double *out, int *nfail, hipStream_t st) {
+                       double diffterm, double *out, int *nfail, hipStream_t st, int nout) {
+  if (nout < 0) nout = n;
   if (nblk <= 0) return GF_OK;
   if (n < 1 || n > 64) return fail(GF_ERR_UNSUPPORTED, "eigenvalue blocks must be 1..64 wide");
   const size_t lds = expeig_lds_bytes(n);
-  // one wave per block; LDS (about 23 KB at n = 53) caps residency near 6 waves per CU, so a grid of
-  // 8 waves per CU keeps every CU busy and lets each wave loop over its share of blocks
-  long grid = (long)(m && m->n_cu > 0 ? m->n_cu : 256) * 8;
+  // one wave per block, persistent: exactly as many workgroups as can be resident at once (LDS-limited, about
+  // 6 per CU at n = 53), each looping over its share of the blocks, so no workgroup waits for a second round
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, expeig_kernel, 64, lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  long grid = (long)(m && m->n_cu > 0 ? m->n_cu : 256) * per_cu;
   if (grid > nblk) grid = nblk;
   hipLaunchKernelGGL(expeig_kernel, dim3((unsigned)grid), dim3(64), lds, st, blocks, (long)n * n, boff, ooff, nblk, n,
-                     diffterm, out, nfail);
+                     diffterm, out, nout, nfail);
   return hipGetLastError() == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "eigenvalue kernel launch failed");
 }
 

@@ -72,23 +72,57 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld,
     const double v0 = x0 - alpha;
     const double beta = -1. / (alpha * v0);
     par.each(0, n, [&](int t) { vbuf[t] = t == k + 1 ? v0 : (t > k + 1 ? a[t * ld + k] : 0.); });
-    // left: rows k+1..n-1 of columns k+1..n-1 (column k becomes alpha e1 exactly)
+    // left: rows k+1..n-1 of columns k+1..n-1 (column k becomes alpha e1 exactly); four rows per trip so that
+    // the loads of a trip are in flight together
     par.each(k + 1, n, [&](int j) {
-      double w = 0.;
-      for (int i = k + 1; i < n; ++i) w += vbuf[i] * a[i * ld + j];
-      w *= beta;
-      for (int i = k + 1; i < n; ++i) a[i * ld + j] -= vbuf[i] * w;
+      double w0 = 0., w1 = 0., w2 = 0., w3 = 0.;
+      int i = k + 1;
+      for (; i + 3 < n; i += 4) {
+        const double x0 = a[i * ld + j], x1 = a[(i + 1) * ld + j], x2 = a[(i + 2) * ld + j], x3 = a[(i + 3) * ld + j];
+        w0 += vbuf[i] * x0;
+        w1 += vbuf[i + 1] * x1;
+        w2 += vbuf[i + 2] * x2;
+        w3 += vbuf[i + 3] * x3;
+      }
+      for (; i < n; ++i) w0 += vbuf[i] * a[i * ld + j];
+      const double w = ((w0 + w1) + (w2 + w3)) * beta;
+      i = k + 1;
+      for (; i + 3 < n; i += 4) {
+        const double x0 = a[i * ld + j], x1 = a[(i + 1) * ld + j], x2 = a[(i + 2) * ld + j], x3 = a[(i + 3) * ld + j];
+        a[i * ld + j] = x0 - vbuf[i] * w;
+        a[(i + 1) * ld + j] = x1 - vbuf[i + 1] * w;
+        a[(i + 2) * ld + j] = x2 - vbuf[i + 2] * w;
+        a[(i + 3) * ld + j] = x3 - vbuf[i + 3] * w;
+      }
+      for (; i < n; ++i) a[i * ld + j] -= vbuf[i] * w;
     });
     par.each(0, n, [&](int t) {
       if (t == k + 1) a[t * ld + k] = alpha;
       else if (t > k + 1) a[t * ld + k] = 0.;
     });
-    // right: all rows, columns k+1..n-1
+    // right: all rows, columns k+1..n-1 (four columns per trip)
     par.each(0, n, [&](int i) {
-      double u = 0.;
-      for (int j = k + 1; j < n; ++j) u += a[i * ld + j] * vbuf[j];
-      u *= beta;
-      for (int j = k + 1; j < n; ++j) a[i * ld + j] -= u * vbuf[j];
+      double u0 = 0., u1 = 0., u2 = 0., u3 = 0.;
+      double *ai = a + i * ld;
+      int j = k + 1;
+      for (; j + 3 < n; j += 4) {
+        const double x0 = ai[j], x1 = ai[j + 1], x2 = ai[j + 2], x3 = ai[j + 3];
+        u0 += x0 * vbuf[j];
+        u1 += x1 * vbuf[j + 1];
+        u2 += x2 * vbuf[j + 2];
+        u3 += x3 * vbuf[j + 3];
+      }
+      for (; j < n; ++j) u0 += ai[j] * vbuf[j];
+      const double u = ((u0 + u1) + (u2 + u3)) * beta;
+      j = k + 1;
+      for (; j + 3 < n; j += 4) {
+        const double x0 = ai[j], x1 = ai[j + 1], x2 = ai[j + 2], x3 = ai[j + 3];
+        ai[j] = x0 - u * vbuf[j];
+        ai[j + 1] = x1 - u * vbuf[j + 1];
+        ai[j + 2] = x2 - u * vbuf[j + 2];
+        ai[j + 3] = x3 - u * vbuf[j + 3];
+      }
+      for (; j < n; ++j) ai[j] -= u * vbuf[j];
     });
   }
 

@@ -19,6 +19,7 @@
 // the CPU.
 #pragma once
 #include <cmath>
+#include <type_traits>
 
 #if defined(__HIPCC__)
 #define GF_HD __host__ __device__
@@ -28,9 +29,11 @@
 
 namespace gfa {
 
-// Matrix storage: row-major a[i * ld + j], ld odd (fewer LDS bank conflicts for column walks).
-template <class P>
-GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld, int *failed) {
+// Matrix storage: row-major a[i * ld + j], ld odd (fewer LDS bank conflicts for column walks). The stride may
+// be an int or a compile-time constant (std::integral_constant), which lets the device fold row offsets into
+// the LDS instructions.
+template <class P, class LD>
+GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, LD ld, int *failed) {
   using std::fabs;
   using std::sqrt;
   *failed = 0;
@@ -196,7 +199,7 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld,
         pp = (rr * ss - ww) / a[(mm + 1) * ld + mm] + a[mm * ld + mm + 1];
         qq = a[(mm + 1) * ld + mm + 1] - z - rr - ss;
         rr3 = a[(mm + 2) * ld + mm + 1];
-        const double is = 1. / (fabs(pp) + fabs(qq) + fabs(rr3));
+        const double is = par.recip(fabs(pp) + fabs(qq) + fabs(rr3));
         pp *= is;
         qq *= is;
         rr3 *= is;
@@ -224,7 +227,7 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld,
           r = k != nn - 1 ? a[(k + 2) * ld + k - 1] : 0.;
           xk = fabs(p) + fabs(q) + fabs(r);
           if (xk != 0.) {
-            const double ix = 1. / xk;
+            const double ix = par.recip(xk);
             p *= ix;
             q *= ix;
             r *= ix;
@@ -233,7 +236,7 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld,
         const double sn = sqrt(p * p + q * q + r * r);
         const double s = p >= 0. ? sn : -sn;
         if (s == 0.) continue;
-        const double is = 1. / s, ips = 1. / (p + s);
+        const double is = par.recip(s), ips = par.recip(p + s);
         const double hx = (p + s) * is, hy = q * is, hz = r * is;
         const double qq = q * ips, rr = r * ips;
         const bool three = k != nn - 1;
@@ -271,6 +274,7 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, int ld,
 // Host policy: lane loops run sequentially (used by the CPU unit tests of this file).
 struct HostLanes {
   template <class F> void each(int lo, int hi, F f) { for (int t = lo; t < hi; ++t) f(t); }
+  double recip(double x) { return 1. / x; }
   // the largest i in [lo, hi) with pred(i), or lo - 1 if none
   template <class F> int last_true(int lo, int hi, F pred) {
     for (int t = hi - 1; t >= lo; --t)

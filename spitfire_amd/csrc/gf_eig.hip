@@ -4,6 +4,8 @@
 #include "gf_wave.h"
 #include "../../include/griffon_amd.h"
 
+#include <type_traits>
+
 namespace gfa {
 
 // Device policy for eig_max_real: one 64-lane wave per matrix, lane t owns row/column t.
@@ -19,6 +21,13 @@ struct WaveLanes {
     const unsigned long long mask = __ballot(v);
     return mask ? 63 - __clzll((long long)mask) : lo - 1;
   }
+  // 1/x from the hardware reciprocal and one Newton step (within an ulp or so; the reflector coefficients
+  // need no more), keeping 1/0 = inf and 1/inf = 0
+  __device__ double recip(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    const double e = fma(-x, r, 1.);
+    return isfinite(e) ? fma(r, e, r) : r;
+  }
   template <class F> __device__ double sum(int lo, int hi, F f) {
     // DPP scan (no LDS permutes); the same value in every lane
     return wave::dpp_wave_sum((lane >= lo && lane < hi) ? f(lane) : 0.);
@@ -29,12 +38,13 @@ struct WaveLanes {
 // (flamelet_kernels.cpp:1329-1341, which writes it once per point unknown: nout = n). Blocks are column-major
 // n x n at blocks + b*bstride. With boff/ooff given, block b starts at blocks + boff[b] and its outputs at
 // out + ooff[b].
+template <int LD>
 __global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ blocks, long bstride,
                                                     const long *__restrict__ boff, const long *__restrict__ ooff,
                                                     long nblk, int n, double diffterm, double *__restrict__ out,
                                                     int nout, int *__restrict__ nfail) {
   extern __shared__ double smem[];
-  const int ld = n | 1;
+  const std::integral_constant<int, LD> ld{};  // row stride of the LDS matrix, odd and >= n
   double *a = smem;
   double *vb = smem + (long)n * ld;
   WaveLanes par{(int)threadIdx.x};
@@ -52,24 +62,49 @@ __global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ b
   }
 }
 
-size_t expeig_lds_bytes(int n) { return ((size_t)n * (n | 1) + 64) * sizeof(double); }
+// Row strides the kernel is compiled for (odd, so column walks are free of bank conflicts); a block of size n
+// uses the smallest one not below n | 1.
+constexpr int kEigStrides[] = {9, 17, 25, 33, 41, 49, 53, 57, 61, 65};
+int eig_stride(int n) {
+  for (int s : kEigStrides)
+    if (s >= (n | 1)) return s;
+  return 65;
+}
+size_t expeig_lds_bytes(int n) { return ((size_t)n * eig_stride(n) + 64) * sizeof(double); }
+
+template <int LD>
+int launch_expeig_ld(const gf_mech *m, const double *blocks, const long *boff, const long *ooff, long nblk, int n,
+                     double diffterm, double *out, int *nfail, hipStream_t st, int nout) {
+  const size_t lds = expeig_lds_bytes(n);
+  // one wave per block, persistent: exactly as many workgroups as can be resident at once (LDS-limited, about
+  // 7 per CU at n = 53), each looping over its share of the blocks, so no workgroup waits for a second round
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, expeig_kernel<LD>, 64, lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  long grid = (long)(m && m->n_cu > 0 ? m->n_cu : 256) * per_cu;
+  if (grid > nblk) grid = nblk;
+  hipLaunchKernelGGL(expeig_kernel<LD>, dim3((unsigned)grid), dim3(64), lds, st, blocks, (long)n * n, boff, ooff, nblk,
+                     n, diffterm, out, nout, nfail);
+  return hipGetLastError() == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "eigenvalue kernel launch failed");
+}
 
 int launch_expeig_list(const gf_mech *m, const double *blocks, const long *boff, const long *ooff, long nblk, int n,
                        double diffterm, double *out, int *nfail, hipStream_t st, int nout) {
   if (nout < 0) nout = n;
   if (nblk <= 0) return GF_OK;
   if (n < 1 || n > 64) return fail(GF_ERR_UNSUPPORTED, "eigenvalue blocks must be 1..64 wide");
-  const size_t lds = expeig_lds_bytes(n);
-  // one wave per block, persistent: exactly as many workgroups as can be resident at once (LDS-limited, about
-  // 6 per CU at n = 53), each looping over its share of the blocks, so no workgroup waits for a second round
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, expeig_kernel, 64, lds) != hipSuccess || per_cu < 1)
-    per_cu = 1;
-  long grid = (long)(m && m->n_cu > 0 ? m->n_cu : 256) * per_cu;
-  if (grid > nblk) grid = nblk;
-  hipLaunchKernelGGL(expeig_kernel, dim3((unsigned)grid), dim3(64), lds, st, blocks, (long)n * n, boff, ooff, nblk, n,
-                     diffterm, out, nout, nfail);
-  return hipGetLastError() == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "eigenvalue kernel launch failed");
+  switch (eig_stride(n)) {
+    case 9: return launch_expeig_ld<9>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+    case 17: return launch_expeig_ld<17>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+    case 25: return launch_expeig_ld<25>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+    case 33: return launch_expeig_ld<33>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+    case 41: return launch_expeig_ld<41>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+    case 49: return launch_expeig_ld<49>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+    case 53: return launch_expeig_ld<53>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+    case 57: return launch_expeig_ld<57>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+    case 61: return launch_expeig_ld<61>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+    default: return launch_expeig_ld<65>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+  }
 }
 
 int launch_expeig(const gf_mech *m, const double *blocks, long bstride, long nblk, int n, double diffterm, double *out,

@@ -155,11 +155,13 @@ int gf_batch_btddod_inverse_solve(int nsys, const int *nblocks, int bs, const do
 
 /* ---------------- many steady flamelets at once, device-resident ------------------------------------ */
 /* A batch of adiabatic flamelets sharing one mechanism and pressure, solved together on the GPU by the
- * reference's pseudo-transient continuation (Flamelet.steady_solve_psitc, flamelet.py:1487-1703): per
- * flamelet the same Jacobian-refresh rule, eigenvalue-limited local pseudo-time steps, line search and
- * restarts, with all flamelets advancing in lock step and every refresh, factorization, solve and RHS
- * evaluation batched over the flamelets that need it. Replaces a loop of Flamelet.compute_steady_state
- * calls over the flamelets of a library (tabulation.py, one flamelet per dissipation rate).
+ * reference's steady solvers: Newton's method (Flamelet.steady_solve_newton, flamelet.py:1352-1485) and
+ * pseudo-transient continuation (Flamelet.steady_solve_psitc, flamelet.py:1487-1703), per flamelet with the
+ * reference's Jacobian-refresh rules, eigenvalue-limited local pseudo-time steps, line search and restarts,
+ * all flamelets advancing in lock step and every refresh, factorization, solve and RHS evaluation batched
+ * over the flamelets that need it; plus the batched operators of the time-stepping fallback. Together they
+ * replace a loop of Flamelet.compute_steady_state calls (flamelet.py:1705-1778) over the flamelets of a
+ * library (tabulation.py:286-296, one flamelet per dissipation rate).
  *
  * Flamelet f has nzi[f] >= 2 interior points; dz holds its nzi[f]+1 grid spacings and chi its nzi[f]+2
  * dissipation rates (full grid, flamelet.py:458-466), concatenated over f; oxy/fuel hold the ns-long
@@ -194,10 +196,34 @@ int gf_flamelet_batch_set_state(gf_flamelet_batch *b, const double *interior);
 int gf_flamelet_batch_get_state(gf_flamelet_batch *b, double *interior);
 /* flamelet RHS of every flamelet at the current state (same layout as the state) */
 int gf_flamelet_batch_rhs(gf_flamelet_batch *b, double *out);
-/* iterations/converged/residual: per flamelet (may be null); counters (may be null): [0] iterations summed
- * over flamelets, [1] Jacobian refreshes, [2] RHS evaluations, [3] synchronizing solver steps */
-int gf_flamelet_batch_steady_psitc(gf_flamelet_batch *b, const gf_psitc_options *opts, int *iterations,
-                                   int *converged, double *residual, long long *counters);
+/* Pseudo-transient continuation (Flamelet.steady_solve_psitc, flamelet.py:1487-1703) of the flamelets with
+ * select[f] != 0 (every flamelet when select is null), each from its initial guess. iterations/converged/
+ * residual/min_ds: per flamelet (may be null; entries of unselected flamelets are left as they are); min_ds
+ * is the smallest pseudo-time step at the end, the fourth value the reference returns and the first step
+ * of its time-stepping fallback (flamelet.py:1757-1767). counters (may be null): [0] iterations summed over
+ * flamelets, [1] Jacobian refreshes, [2] RHS evaluations, [3] synchronizing solver steps */
+int gf_flamelet_batch_steady_psitc(gf_flamelet_batch *b, const gf_psitc_options *opts, const int *select,
+                                   int *iterations, int *converged, double *residual, double *min_ds,
+                                   long long *counters);
+/* Newton's method with the reference's line search (Flamelet.steady_solve_newton, flamelet.py:1352-1485) for
+ * the selected flamelets, each from its initial guess: the matrix -J is refreshed only after a line search
+ * shortened the step; no restarts. Uses tolerance, max_iterations, max_factor_line_search,
+ * max_allowed_residual, min_allowable_state_var, lu_precision and log_rate of opts. Outputs as for psitc. */
+int gf_flamelet_batch_steady_newton(gf_flamelet_batch *b, const gf_psitc_options *opts, const int *select,
+                                    int *iterations, int *converged, double *residual, long long *counters);
+/* The operators of implicit time stepping (Flamelet.integrate with block-Thomas solves, flamelet.py:741-762,
+ * 1046-1213) for the flamelets with flags[f] != 0, on DEVICE vectors in the dof layout of the batch (flamelet
+ * f's interior state, point-major, from offset sum_{g<f} nzi[g]*ns; NV = sum nzi*ns doubles); entries of the
+ * other flamelets are neither read nor written. Synchronous: the work is complete on return.
+ *   transient_rhs:   rhs = F(x); x also becomes the current state of those flamelets
+ *   transient_setup: factor prefactor[f] * J(x) - I per flagged flamelet (prefactor: host, one per flamelet);
+ *                    lu_precision 64 or 32, the same for every setup while factors are held
+ *   transient_solve: out = (prefactor[f] * J - I)^-1 r with each flamelet's factors from its last
+ *                    transient_setup (an error if it has none, or a steady solve refreshed it since) */
+int gf_flamelet_batch_transient_rhs(gf_flamelet_batch *b, const int *flags, const double *x, double *rhs);
+int gf_flamelet_batch_transient_setup(gf_flamelet_batch *b, const int *flags, const double *x,
+                                      const double *prefactor, int lu_precision);
+int gf_flamelet_batch_transient_solve(gf_flamelet_batch *b, const int *flags, const double *r, double *out);
 
 #ifdef __cplusplus
 }

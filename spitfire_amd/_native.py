@@ -60,7 +60,11 @@ SIGNATURES = {
     'gf_flamelet_batch_set_state': [_V, _D],
     'gf_flamelet_batch_get_state': [_V, _D],
     'gf_flamelet_batch_rhs': [_V, _D],
-    'gf_flamelet_batch_steady_psitc': [_V, _V, _I, _I, _D, ctypes.POINTER(ctypes.c_longlong)],
+    'gf_flamelet_batch_steady_psitc': [_V, _V, _I, _I, _I, _D, _D, ctypes.POINTER(ctypes.c_longlong)],
+    'gf_flamelet_batch_steady_newton': [_V, _V, _I, _I, _I, _D, ctypes.POINTER(ctypes.c_longlong)],
+    'gf_flamelet_batch_transient_rhs': [_V, _I, _V, _V],
+    'gf_flamelet_batch_transient_setup': [_V, _I, _V, _D, _int],
+    'gf_flamelet_batch_transient_solve': [_V, _I, _V, _V],
 }
 
 

@@ -1,10 +1,12 @@
-// Device-resident batched steady flamelet solver: the reference's pseudo-transient continuation
-// (Flamelet.steady_solve_psitc, flamelet.py:1487-1703) run for many flamelets at once. The host keeps each
-// flamelet's small control state (iteration count, Jacobian age, line-search factor, restart depth) and
-// launches every heavy step batched over the flamelets that need it: chemistry Jacobians + per-block
-// eigenvalues + pseudo-transient matrix + block-Thomas factorization for the flamelets due a refresh,
-// block-Thomas solves, trial states, chemistry RHS and residual norms. One host synchronization per
-// line-search round reads back the per-flamelet residual norms.
+// Device-resident batched steady flamelet solvers: the reference's Newton method and pseudo-transient
+// continuation (Flamelet.steady_solve_newton, flamelet.py:1352-1485; Flamelet.steady_solve_psitc,
+// flamelet.py:1487-1703) run for many flamelets at once, plus the batched right-hand sides and implicit-step
+// linear systems that the time-stepping fallback of compute_steady_state (flamelet.py:1759-1778) needs. The
+// host keeps each flamelet's small control state (iteration count, Jacobian age, line-search factor, restart
+// depth) and launches every heavy step batched over the flamelets that need it: chemistry Jacobians (+ per-block
+// eigenvalues and pseudo-time steps for psitc) + the Newton-type matrix + block-Thomas factorization for the
+// flamelets due a refresh, block-Thomas solves, trial states, chemistry RHS and residual norms. One host
+// synchronization per line-search round reads back the per-flamelet residual norms.
 #include "gf_internal.h"
 #include "../../include/griffon_amd.h"
 
@@ -63,6 +65,8 @@ struct gf_flamelet_batch {
   long *dpt0 = nullptr, *ddof0 = nullptr, *dipt0 = nullptr, *dchi0 = nullptr, *dvoff = nullptr, *ddz0 = nullptr,
        *js_off = nullptr, *eb = nullptr, *eo = nullptr, *sys_v = nullptr, *sys_l = nullptr, *sys_p = nullptr,
        *sys_x = nullptr;
+  int tr_prec = 64;          // precision of the transient_setup factorizations
+  std::vector<char> tr_set;  // flamelet f holds transient_setup factors (and its values in J are theirs)
   FlameletBatchView view;
   // pinned host staging for the solver's small uploads; reused only after a stream synchronization
   char *hpin = nullptr;
@@ -134,11 +138,14 @@ size_t carve_all(gf_flamelet_batch *b, char *base) {
   return c.off + 256;
 }
 
-// the solver for one call of gf_flamelet_batch_steady_psitc
+// the solver for one call of gf_flamelet_batch_steady_psitc or gf_flamelet_batch_steady_newton; its helpers
+// (uploads, chemistry, systems) also serve the transient entry points
 struct Psitc {
   gf_flamelet_batch *b;
   gf_psitc_options o;
   hipStream_t st;
+  int newton = 0;         // 1: the reference Newton method instead of pseudo-transient continuation
+  const int *pick = nullptr;  // flamelets to solve (select[f] != 0); null: all
   long long nit = 0, njac = 0, nrhs = 0, nsync = 0;
   size_t cur = 0;  // staging cursor into b->hpin
 
@@ -235,12 +242,26 @@ struct Psitc {
     return up(b->sys_x, x);
   }
 
-  // Jacobian refresh for the flamelets in fl at X: chemistry blocks, eigenvalue limiter, local pseudo-time
-  // steps, diag(1/ds) - J, block-Thomas factorization (flamelet.py:1597-1608)
+  int factor(const std::vector<int> &fl, int precision) {
+    GF_TRY(systems(fl));
+    if (precision == 32)
+      return launch_btd_inv_factor<float>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->D32, b->sys_l, st);
+    return launch_btd_inv_factor<double>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->L, b->sys_l, st);
+  }
+
+  // Jacobian refresh for the flamelets in fl at X. Psitc: chemistry blocks, eigenvalue limiter, local
+  // pseudo-time steps, diag(1/ds) - J, block-Thomas factorization (flamelet.py:1597-1608). Newton: -J,
+  // factored (flamelet.py:1421-1427).
   int refresh(const std::vector<int> &fl, const std::vector<int> &first) {
     if (fl.empty()) return GF_OK;
     njac += (long long)fl.size();
+    for (int f : fl) b->tr_set[f] = 0;  // their values and factors are overwritten
     GF_TRY(chem(b->X, fl, true));
+    if (newton) {
+      GF_TRY(up(b->flag, flags_of(fl)));
+      GF_TRY(launch_fb_matrix(b->m, b->view, b->X, b->flag, 1, nullptr, 0., b->J, st));
+      return factor(fl, o.lu_precision);
+    }
     std::vector<long> eb, eo;
     const long bsz = (long)b->ns * b->ns;
     for (int f : fl)
@@ -257,24 +278,21 @@ struct Psitc {
     for (size_t k = 0; k < fl.size(); ++k) g[fl[k]] = first[k];
     GF_TRY(up(b->gmin, g));
     GF_TRY(launch_fb_ds(b->view, b->flag, b->gmin, b->eig, b->fdiff, o.ds_safety, o.ds_ramp, o.ds_max, b->ds, st));
-    GF_TRY(launch_fb_jac_psitc(b->m, b->view, b->X, b->flag, b->ds, b->J, st));
-    GF_TRY(systems(fl));
-    if (o.lu_precision == 32)
-      return launch_btd_inv_factor<float>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->D32, b->sys_l, st);
-    return launch_btd_inv_factor<double>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->L, b->sys_l, st);
+    GF_TRY(launch_fb_matrix(b->m, b->view, b->X, b->flag, 0, b->ds, 0., b->J, st));
+    return factor(fl, o.lu_precision);
   }
 
-  int solve(const std::vector<int> &fl) {
+  int solve(const std::vector<int> &fl, int precision) {
     if (fl.empty()) return GF_OK;
     GF_TRY(systems(fl));
-    if (o.lu_precision == 32)
+    if (precision == 32)
       return launch_btd_inv_solve<float>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->D32, b->sys_l, b->R,
                                          b->dX, b->sys_x, st);
     return launch_btd_inv_solve<double>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->L, b->sys_l, b->R, b->dX,
                                         b->sys_x, st);
   }
 
-  int run(int *iterations, int *converged, double *residual) {
+  int run(int *iterations, int *converged, double *residual, double *min_ds) {
     const int B = b->B;
     std::vector<int> active(B, 1), it(B, 0), age(B, 0), evalJ(B, 1), depth(B, 0), conv(B, 0);
     std::vector<double> dsinit(B, o.ds_init), res(B, 0.), nrm(B, 0.), alpha(B, 1.), nold(B, 0.), tmp, tmin;
@@ -284,10 +302,12 @@ struct Psitc {
       if (fl.empty()) return GF_OK;
       GF_TRY(up(b->flag, flags_of(fl)));
       GF_TRY(launch_fb_copy_interior(b->view, b->flag, b->X0, b->X, st));
-      std::vector<double> v(B, 0.);
-      for (int f : fl) v[f] = dsinit[f];
-      GF_TRY(up(b->fval, v));
-      GF_TRY(launch_fb_fill_dofs(b->view, b->flag, b->fval, b->ds, st));
+      if (!newton) {
+        std::vector<double> v(B, 0.);
+        for (int f : fl) v[f] = dsinit[f];
+        GF_TRY(up(b->fval, v));
+        GF_TRY(launch_fb_fill_dofs(b->view, b->flag, b->fval, b->ds, st));
+      }
       GF_TRY(rhs(b->X, fl, b->R, tmp, nullptr));
       for (int f : fl) {
         nrm[f] = tmp[f];
@@ -298,11 +318,12 @@ struct Psitc {
       }
       return GF_OK;
     };
-    // restart with a smaller initial pseudo-time step, or give up (flamelet.py:1563-1571)
+    // psitc: restart with a smaller initial pseudo-time step, or give up (flamelet.py:1563-1571); Newton
+    // gives up at once (flamelet.py:1435-1468)
     auto restart = [&](const std::vector<int> &fl) -> int {
       std::vector<int> again;
       for (int f : fl) {
-        if (depth[f] > o.max_recursion_depth || !o.adaptive_restart) {
+        if (newton || depth[f] > o.max_recursion_depth || !o.adaptive_restart) {
           active[f] = 0;
           conv[f] = 0;
         } else {
@@ -315,7 +336,10 @@ struct Psitc {
     };
 
     std::vector<int> all;
-    for (int f = 0; f < B; ++f) all.push_back(f);
+    for (int f = 0; f < B; ++f) {
+      if (!pick || pick[f]) all.push_back(f);
+      else active[f] = 0;
+    }
     GF_TRY(start(all));
     const auto t0 = std::chrono::steady_clock::now();
     long sweep = 0;
@@ -339,8 +363,8 @@ struct Psitc {
         double rmax = 0.;
         for (int f : step) rmax = std::max(rmax, res[f]);
         const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        std::fprintf(stderr, "psitc sweep %ld: %zu of %d flamelets active, max residual %.3e, %.2f s\n", sweep - 1,
-                     step.size(), B, rmax, el);
+        std::fprintf(stderr, "%s sweep %ld: %zu of %d flamelets active, max residual %.3e, %.2f s\n",
+                     newton ? "newton" : "psitc", sweep - 1, step.size(), B, rmax, el);
         std::fflush(stderr);
       }
       std::vector<int> ref, first;
@@ -354,10 +378,12 @@ struct Psitc {
         } else {
           age[f] += 1;
         }
-        evalJ[f] = (age[f] == 8) || res[f] > 1.e-2;
+        // psitc refreshes every 8 iterations and while the residual is large (flamelet.py:1615); Newton keeps
+        // its Jacobian until a line search cuts the step (flamelet.py:1420-1449)
+        evalJ[f] = newton ? 0 : ((age[f] == 8) || res[f] > 1.e-2);
       }
       GF_TRY(refresh(ref, first));
-      GF_TRY(solve(step));
+      GF_TRY(solve(step, o.lu_precision));
       // full step, then the line search on the scaled residual (flamelet.py:1623-1650)
       GF_TRY(up(b->flag, flags_of(step)));
       GF_TRY(launch_fb_trial(b->view, b->flag, nullptr, b->X, b->Xt, b->dX, st));
@@ -406,12 +432,31 @@ struct Psitc {
     }
     GF_TRY(launch_fb_clip_points(b->NP * b->ns, b->X, st));
     GF_TRY(hipStreamSynchronize(st) == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "stream synchronize failed"));
-    for (int f = 0; f < B; ++f) {
+    std::vector<double> dsh;
+    if (min_ds && !newton) GF_TRY(down(dsh, b->ds, b->NV));
+    for (int f : all) {
       if (iterations) iterations[f] = it[f];
       if (converged) converged[f] = conv[f];
       if (residual) residual[f] = res[f];
+      if (min_ds && !newton) {  // np.min(ds) as the reference returns it (flamelet.py:1693-1703)
+        double mn = INFINITY;
+        for (long e = 0; e < (long)b->nzi[f] * b->ns; ++e) mn = std::min(mn, dsh[b->dof0[f] + e]);
+        min_ds[f] = mn;
+      }
     }
     return GF_OK;
+  }
+
+  // the flamelets with flags[f] != 0, uploaded as the device flags
+  int flagged(const int *flags, std::vector<int> &fl) {
+    for (int f = 0; f < b->B; ++f)
+      if (flags[f]) fl.push_back(f);
+    return fl.empty() ? GF_OK : up(b->flag, flags_of(fl));
+  }
+
+  int sync() {
+    cur = 0;
+    return hipStreamSynchronize(st) == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "stream synchronize failed");
   }
 };
 
@@ -449,6 +494,7 @@ int gf_flamelet_batch_create(gf_mech *m, int nfl, const int *nzi, double pressur
   auto *b = new gf_flamelet_batch();
   b->m = m;
   b->B = nfl;
+  b->tr_set.assign(nfl, 0);
   b->ns = ns;
   b->pressure = pressure;
   long P = 0, V = 0, I = 0, C = 0, Jv = 0, Lv = 0, Pv = 0, Z = 0;
@@ -601,8 +647,9 @@ int gf_flamelet_batch_rhs(gf_flamelet_batch *b, double *out) {
   return GF_OK;
 }
 
-int gf_flamelet_batch_steady_psitc(gf_flamelet_batch *b, const gf_psitc_options *opts, int *iterations,
-                                   int *converged, double *residual, long long *counters) {
+namespace {
+int steady(gf_flamelet_batch *b, const gf_psitc_options *opts, int newton, const int *select, int *iterations,
+           int *converged, double *residual, double *min_ds, long long *counters) {
   if (!b) return fail(GF_ERR_ARG, "null flamelet batch");
   gf_psitc_options o;
   gf_psitc_options_default(&o);
@@ -612,7 +659,9 @@ int gf_flamelet_batch_steady_psitc(gf_flamelet_batch *b, const gf_psitc_options 
   for (int f = 0; f < b->B; ++f) d[f] = o.diffusion_factor * b->diffmax[f];
   GF_TRY(up(b->fdiff, d, b->m->stream));
   Psitc s{b, o, b->m->stream};
-  GF_TRY(s.run(iterations, converged, residual));
+  s.newton = newton;
+  s.pick = select;
+  GF_TRY(s.run(iterations, converged, residual, min_ds));
   if (counters) {
     counters[0] = s.nit;
     counters[1] = s.njac;
@@ -620,6 +669,70 @@ int gf_flamelet_batch_steady_psitc(gf_flamelet_batch *b, const gf_psitc_options 
     counters[3] = s.nsync;
   }
   return GF_OK;
+}
+}  // namespace
+
+int gf_flamelet_batch_steady_psitc(gf_flamelet_batch *b, const gf_psitc_options *opts, const int *select,
+                                   int *iterations, int *converged, double *residual, double *min_ds,
+                                   long long *counters) {
+  return steady(b, opts, 0, select, iterations, converged, residual, min_ds, counters);
+}
+
+int gf_flamelet_batch_steady_newton(gf_flamelet_batch *b, const gf_psitc_options *opts, const int *select,
+                                    int *iterations, int *converged, double *residual, long long *counters) {
+  return steady(b, opts, 1, select, iterations, converged, residual, nullptr, counters);
+}
+
+int gf_flamelet_batch_transient_rhs(gf_flamelet_batch *b, const int *flags, const double *x, double *rhs) {
+  if (!b || !flags || !x || !rhs) return fail(GF_ERR_ARG, "null argument");
+  Psitc s{b, {}, b->m->stream};
+  std::vector<int> fl;
+  GF_TRY(s.flagged(flags, fl));
+  if (fl.empty()) return GF_OK;
+  GF_TRY(launch_fb_unpack(b->view, b->flag, b->ddof0, x, b->X, 1, s.st));
+  GF_TRY(s.chem(b->X, fl, false));
+  GF_TRY(launch_fb_rhs(b->view, b->X, b->flag, rhs, s.st));
+  return s.sync();
+}
+
+int gf_flamelet_batch_transient_setup(gf_flamelet_batch *b, const int *flags, const double *x,
+                                      const double *prefactor, int lu_precision) {
+  if (!b || !flags || !x || !prefactor) return fail(GF_ERR_ARG, "null argument");
+  if (lu_precision != 64 && lu_precision != 32) return fail(GF_ERR_ARG, "lu_precision must be 64 or 32");
+  if (b->tr_prec != lu_precision && std::any_of(b->tr_set.begin(), b->tr_set.end(), [](char c) { return c != 0; }))
+    return fail(GF_ERR_STATE, "transient_setup precision differs from the factors already held");
+  Psitc s{b, {}, b->m->stream};
+  std::vector<int> fl;
+  GF_TRY(s.flagged(flags, fl));
+  if (fl.empty()) return GF_OK;
+  GF_TRY(launch_fb_unpack(b->view, b->flag, b->ddof0, x, b->X, 1, s.st));
+  GF_TRY(s.chem(b->X, fl, true));
+  GF_TRY(s.up(b->fval, std::vector<double>(prefactor, prefactor + b->B)));
+  GF_TRY(launch_fb_matrix(b->m, b->view, b->X, b->flag, 2, b->fval, 0., b->J, s.st));
+  GF_TRY(s.factor(fl, lu_precision));
+  b->tr_prec = lu_precision;
+  for (int f : fl) b->tr_set[f] = 1;
+  return s.sync();
+}
+
+int gf_flamelet_batch_transient_solve(gf_flamelet_batch *b, const int *flags, const double *r, double *out) {
+  if (!b || !flags || !r || !out) return fail(GF_ERR_ARG, "null argument");
+  Psitc s{b, {}, b->m->stream};
+  std::vector<int> fl;
+  for (int f = 0; f < b->B; ++f)
+    if (flags[f]) {
+      if (!b->tr_set[f]) return fail(GF_ERR_STATE, "transient_solve before transient_setup for a flamelet");
+      fl.push_back(f);
+    }
+  if (fl.empty()) return GF_OK;
+  GF_TRY(s.systems(fl));
+  if (b->tr_prec == 32)
+    GF_TRY(launch_btd_inv_solve<float>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->D32, b->sys_l, r, out,
+                                       b->sys_x, s.st));
+  else
+    GF_TRY(launch_btd_inv_solve<double>((int)fl.size(), b->sys_nb, b->ns, b->J, b->sys_v, b->L, b->sys_l, r, out,
+                                        b->sys_x, s.st));
+  return s.sync();
 }
 
 }  // extern "C"

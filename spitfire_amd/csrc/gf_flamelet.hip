@@ -272,10 +272,15 @@ __global__ void fb_rhs_kernel(FlameletBatchView b, const double *X, const int *f
   rhs_entry(io, e - b.dof0[f], R + b.dof0[f]);
 }
 
-// the pseudo-transient matrix diag(1/ds) - J of each flagged flamelet (flamelet.py:1597-1606: a fresh
-// Jacobian, then btddod_scale_and_add_diagonal(J, -1, 1/ds, 1)); the chemistry blocks are already in J
-__global__ void fb_jac_psitc_kernel(MechView mv, FlameletBatchView b, const double *X, const int *flag,
-                                    const double *ds, double *J) {
+// The Newton-type matrix of each flagged flamelet from a fresh Jacobian (the chemistry blocks are already in J):
+//   mode 0  diag(1/ds) - J     pseudo-transient continuation (flamelet.py:1597-1606:
+//                              btddod_scale_and_add_diagonal(J, -1, 1/ds, 1))
+//   mode 1  -J                 Newton (flamelet.py:1422)
+//   mode 2  prefactor * J - I  implicit time stepping (flamelet.py:756-762, flamelet_kernels.cpp:1395-1408), with
+//                              the prefactor of flamelet f in vals[f] (or the scalar prefactor when vals is null)
+// vals: mode 0 the local pseudo-time steps (dof layout), mode 2 per-flamelet prefactors
+__global__ void fb_matrix_kernel(MechView mv, FlameletBatchView b, const double *X, const int *flag, int mode,
+                                 const double *vals, double prefactor, double *J) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= b.NJ) return;
   const int f = fb_find(b.voff, b.B, e);
@@ -284,10 +289,22 @@ __global__ void fb_jac_psitc_kernel(MechView mv, FlameletBatchView b, const doub
   const long el = e - b.voff[f];
   double v = el < (long)io.nzi * io.ns * io.ns ? J[e] : 0.;
   int dr;
-  jac_entry(mv, io, el, 0, 0., 0, v, dr);
-  double out = -v;
-  if (dr >= 0) out += 1. / ds[b.dof0[f] + dr];
+  jac_entry(mv, io, el, mode == 2, mode == 2 && vals ? vals[f] : prefactor, 0, v, dr);
+  double out = mode == 2 ? v : -v;
+  if (mode == 0 && dr >= 0) out += 1. / vals[b.dof0[f] + dr];
   J[e] = out;
+}
+
+// Copies the flagged flamelets' unknowns from a vector holding flamelet f's unknowns from pk[f] into the batch
+// layout: the dof layout (unknowns at dof0[f]) or, when interior is set, the padded interior rows of X
+__global__ void fb_unpack_kernel(FlameletBatchView b, const int *flag, const long *pk, const double *packed,
+                                 double *dst, int interior) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= b.NV) return;
+  const int f = fb_find(b.dof0, b.B, e);
+  if (!flag[f]) return;
+  const long el = e - b.dof0[f];
+  dst[interior ? (b.pt0[f] + 1) * b.ns + el : e] = packed[pk[f] + el];
 }
 
 // per flagged flamelet: max |R * 1/scale| (T scaled by 1e3, flamelet.py variable scales; NaN if any entry is
@@ -450,10 +467,15 @@ int launch_fb_stencils(const FlameletBatchView &b, const double *dz, const long 
 int launch_fb_rhs(const FlameletBatchView &b, const double *X, const int *flag, double *R, hipStream_t st) {
   FB_LAUNCH(fb_rhs_kernel, b.NV, b, X, flag, R);
 }
-int launch_fb_jac_psitc(const gf_mech *m, const FlameletBatchView &b, const double *X, const int *flag,
-                        const double *ds, double *J, hipStream_t st) {
-  FB_LAUNCH(fb_jac_psitc_kernel, b.NJ, m->view, b, X, flag, ds, J);
+int launch_fb_matrix(const gf_mech *m, const FlameletBatchView &b, const double *X, const int *flag, int mode,
+                     const double *ds, double prefactor, double *J, hipStream_t st) {
+  FB_LAUNCH(fb_matrix_kernel, b.NJ, m->view, b, X, flag, mode, ds, prefactor, J);
 }
+int launch_fb_unpack(const FlameletBatchView &b, const int *flag, const long *pk, const double *packed, double *dst,
+                     int interior, hipStream_t st) {
+  FB_LAUNCH(fb_unpack_kernel, b.NV, b, flag, pk, packed, dst, interior);
+}
+
 int launch_fb_trial(const FlameletBatchView &b, const int *flag, const double *scale, const double *X, double *Xt,
                     double *dX, hipStream_t st) {
   FB_LAUNCH(fb_trial_kernel, b.NV, b, flag, scale, X, Xt, dX);

@@ -190,8 +190,14 @@ struct FlameletBatchView {
 int launch_fb_stencils(const FlameletBatchView &b, const double *dz, const long *dz0, double *cmaj, double *csub,
                        double *csup, double *mco, double *nco, hipStream_t st);
 int launch_fb_rhs(const FlameletBatchView &b, const double *X, const int *flag, double *R, hipStream_t st);
-int launch_fb_jac_psitc(const gf_mech *m, const FlameletBatchView &b, const double *X, const int *flag,
-                        const double *ds, double *J, hipStream_t st);
+// mode 0: diag(1/ds) - J (pseudo-transient, vals = ds); 1: -J (Newton); 2: prefactor * J - I (implicit time
+// stepping, vals = per-flamelet prefactors, or the scalar prefactor when null)
+int launch_fb_matrix(const gf_mech *m, const FlameletBatchView &b, const double *X, const int *flag, int mode,
+                     const double *vals, double prefactor, double *J, hipStream_t st);
+// flagged flamelets' unknowns from a vector holding flamelet f's unknowns from pk[f] into the dof layout, or
+// into the padded interior rows of X when interior is set
+int launch_fb_unpack(const FlameletBatchView &b, const int *flag, const long *pk, const double *packed, double *dst,
+                     int interior, hipStream_t st);
 int launch_fb_norms(const FlameletBatchView &b, const int *flag, const double *R, const double *X, double *res,
                     double *xmin, hipStream_t st);
 int launch_fb_trial(const FlameletBatchView &b, const int *flag, const double *scale, const double *X, double *Xt,

@@ -11,7 +11,8 @@ Differences from the reference, all documented here:
   * the pseudo-transient eigenvalue estimate `expeig` (flamelet.py:1597-1604; LAPACK dgeev per block in
     the reference) comes from the GPU eigenvalue kernel (balancing + Householder Hessenberg reduction +
     Francis double-shift QR), which agrees with dgeev to rounding, not bitwise;
-  * compute_steady_state falls back from Newton to psitc; the ESDIRK fallback is not built.
+  * none in compute_steady_state: Newton, then psitc, then ESDIRK64 time stepping, as the reference; the
+    FlameletBatch form runs each stage batched over the flamelets still unconverged.
 """
 import numpy as np
 from scipy.special import erfinv
@@ -458,6 +459,17 @@ class Flamelet:
         return self._z.copy()
 
     @property
+    def linear_temperature(self):
+        """T_oxy + (T_fuel - T_oxy) z on the full grid (flamelet.py:971-976)"""
+        To, Tf = self._oxy_stream.T, self._fuel_stream.T
+        return To + (Tf - To) * self._z
+
+    @property
+    def current_temperature(self):
+        """temperature of the current state on the full grid, boundary streams included (flamelet.py:1007-1010)"""
+        return np.hstack((self._state_oxy[0], self._current_state[::self._n_equations], self._state_fuel[0]))
+
+    @property
     def dissipation_rate(self):
         return self._x.copy()
 
@@ -549,11 +561,13 @@ class FlameletBatch:
         return int(self._offsets[-1])
 
     def set_state(self, interior):
+        """current state and initial guess of every flamelet (interior states, concatenated)"""
         from ._native import check, dptr
         x = np.ascontiguousarray(interior, dtype=np.float64)
         if x.size != self.n_unknowns:
             raise ValueError('state size %d, expected %d' % (x.size, self.n_unknowns))
         check(self._lib.gf_flamelet_batch_set_state(self._h, dptr(x)))
+        self._x0 = x.copy()
 
     def get_state(self):
         from ._native import check, dptr
@@ -571,48 +585,293 @@ class FlameletBatch:
         check(self._lib.gf_flamelet_batch_rhs(self._h, dptr(out)))
         return out
 
-    def compute_steady_state(self, tolerance=1.e-6, lu_precision=64, **psitc_args):
-        """Pseudo-transient continuation of every flamelet from its current state (psitc_args: the keyword
-        arguments of Flamelet.steady_solve_psitc). Returns (states, converged, iterations); raises if any
-        flamelet fails, unless psitc_args['raise_on_failure'] is False."""
-        from ._native import check, iptr, dptr, PsitcOptions
+    _OPTION_NAMES = ('max_iterations', 'ds_init', 'ds_init_decrease', 'adaptive_restart', 'max_recursion_depth',
+                     'diffusion_factor', 'global_ds', 'ds_safety', 'ds_ramp', 'ds_max', 'max_factor_line_search',
+                     'max_allowed_residual', 'min_allowable_state_var')
+
+    def _options(self, tolerance, lu_precision, args, allowed):
+        from ._native import PsitcOptions
         import ctypes
-        raise_on_failure = psitc_args.pop('raise_on_failure', True)
         o = PsitcOptions()
         self._lib.gf_psitc_options_default(ctypes.byref(o))
         o.tolerance = tolerance
         o.lu_precision = int(lu_precision)
-        names = {'max_iterations': 'max_iterations', 'ds_init': 'ds_init', 'ds_init_decrease': 'ds_init_decrease',
-                 'adaptive_restart': 'adaptive_restart', 'max_recursion_depth': 'max_recursion_depth',
-                 'diffusion_factor': 'diffusion_factor', 'global_ds': 'global_ds', 'ds_safety': 'ds_safety',
-                 'ds_ramp': 'ds_ramp', 'ds_max': 'ds_max', 'max_factor_line_search': 'max_factor_line_search',
-                 'max_allowed_residual': 'max_allowed_residual',
-                 'min_allowable_state_var': 'min_allowable_state_var'}
-        for k, v in psitc_args.items():
+        for k, v in args.items():
             if k == 'log_rate':
                 o.log_rate = int(v)
-                continue
-            if k in ('verbose', 'norm_order'):
-                continue
-            if k not in names:
-                raise TypeError('unknown psitc argument ' + k)
-            setattr(o, names[k], type(getattr(o, names[k]))(v))
+            elif k in ('verbose', 'norm_order', 'initial_guess'):
+                if k == 'initial_guess' and v is not None:
+                    raise TypeError('the batch solvers start from the initial guesses given to set_state')
+            elif k in allowed:
+                setattr(o, k, type(getattr(o, k))(v))
+            else:
+                raise TypeError('unknown solver argument ' + k)
+        return o
+
+    def _select(self, select):
+        B = len(self._flamelets)
+        if select is None:
+            return np.ones(B, dtype=np.int32)
+        s = np.ascontiguousarray(np.asarray(select).astype(bool).astype(np.int32))
+        if s.size != B:
+            raise ValueError('select has %d entries for %d flamelets' % (s.size, B))
+        return s
+
+    def _finish(self, sel, conv, it, raise_on_failure, what):
+        """write converged states back to their Flamelets; (states, converged, iterations) over the batch"""
+        states = self.split(self.get_state())
+        for k in np.flatnonzero(sel):
+            if conv[k]:
+                self._flamelets[k]._current_state = states[k].copy()
+                self._flamelets[k]._iteration_count = int(it[k])
+        if raise_on_failure and not np.all(conv[sel.astype(bool)]):
+            raise RuntimeError('%s did not converge for %d of %d flamelets'
+                               % (what, int(sel.sum() - conv[sel.astype(bool)].sum()), int(sel.sum())))
+        return states, conv.astype(bool), it
+
+    def _steady(self, newton, tolerance, lu_precision, select, args):
+        from ._native import check, iptr, dptr
+        import ctypes
+        raise_on_failure = args.pop('raise_on_failure', True)
+        allowed = (('max_iterations', 'max_factor_line_search', 'max_allowed_residual', 'min_allowable_state_var')
+                   if newton else self._OPTION_NAMES)
+        if newton:
+            args.setdefault('max_iterations', 10)  # steady_solve_newton's default (flamelet.py:1355)
+        o = self._options(tolerance, lu_precision, args, allowed)
+        sel = self._select(select)
         B = len(self._flamelets)
         it = np.zeros(B, dtype=np.int32)
         conv = np.zeros(B, dtype=np.int32)
-        res = np.zeros(B)
+        res = np.full(B, np.nan)
         cnt = (ctypes.c_longlong * 4)()
-        check(self._lib.gf_flamelet_batch_steady_psitc(self._h, ctypes.byref(o), iptr(it), iptr(conv), dptr(res),
-                                                       cnt))
+        if newton:
+            check(self._lib.gf_flamelet_batch_steady_newton(self._h, ctypes.byref(o), iptr(sel), iptr(it), iptr(conv),
+                                                            dptr(res), cnt))
+        else:
+            self.min_ds = np.full(B, np.nan)
+            check(self._lib.gf_flamelet_batch_steady_psitc(self._h, ctypes.byref(o), iptr(sel), iptr(it), iptr(conv),
+                                                           dptr(res), dptr(self.min_ds), cnt))
         self.counters = {'iterations': cnt[0], 'jacobian_refreshes': cnt[1], 'rhs_evaluations': cnt[2],
                          'synchronizing_steps': cnt[3]}
-        states = self.split(self.get_state())
-        for f, s, c, n in zip(self._flamelets, states, conv, it):
-            if c:
-                f._current_state = s.copy()
-                f._iteration_count = int(n)
         self.residuals = res
+        return self._finish(sel, conv, it, raise_on_failure,
+                            'steady flamelet Newton solve' if newton else 'steady flamelet solve')
+
+    def steady_solve_newton(self, tolerance=1.e-6, lu_precision=64, select=None, **newton_args):
+        """Flamelet.steady_solve_newton (flamelet.py:1352-1485) for the selected flamelets (all by default, or
+        a boolean mask), each from its initial guess; newton_args: max_iterations (10), max_factor_line_search,
+        max_allowed_residual, min_allowable_state_var, log_rate, raise_on_failure (True). Returns (states,
+        converged, iterations) over the whole batch; unselected flamelets report not converged."""
+        return self._steady(True, tolerance, lu_precision, select, dict(newton_args))
+
+    def steady_solve_psitc(self, tolerance=1.e-6, lu_precision=64, select=None, **psitc_args):
+        """Pseudo-transient continuation (Flamelet.steady_solve_psitc, flamelet.py:1487-1703) of the selected
+        flamelets from their initial guesses (psitc_args: its keyword arguments, plus raise_on_failure, True
+        by default). Returns (states, converged, iterations); the smallest final pseudo-time step of each
+        flamelet is left in min_ds."""
+        return self._steady(False, tolerance, lu_precision, select, dict(psitc_args))
+
+    def integrate_to_steady(self, steady_tolerance=1.e-4, lu_precision=64, select=None, first_time_step=1.e-6,
+                            max_time_step=1.e-3, minimum_time_step_count=40, transient_tolerance=1.e-10,
+                            maximum_steps_per_jacobian=10, nonlinear_solve_tolerance=1.e-12, max_nonlinear_iter=20,
+                            maximum_time_step_count=None, log_rate=0, raise_on_failure=True, **unused):
+        """Flamelet.integrate_to_steady (flamelet.py:1215-1237; Flamelet.integrate with block-Thomas solves,
+        flamelet.py:1046-1213; same defaults) for the selected flamelets, each from its initial guess, all on the
+        GPU at once. Every flamelet keeps its own time, step size, stage Newton convergence, Jacobian-refresh
+        state and steadiness test, exactly as the reference's step loop (integrator.py:100-130, 229-713) runs
+        for one flamelet: ESDIRK64 stages solved by the frozen-Jacobian Newton method (nonlinear.py:185-268) with
+        block-Thomas solves of dt*gamma*J - I, PI step control (stepcontrol.py:50-117), negative entries clipped
+        after every step, and a stop when |dx * 1/scale|_inf / dt < steady_tolerance. first_time_step: a scalar
+        or one value per flamelet. Returns (states, converged, steps); converged where that test stopped the
+        flamelet (not the step limit)."""
+        import ctypes
+        import sys
+        import torch
+        from ._native import check, iptr
+        from .time_integration import KennedyCarpenterS6P4Q3, SimpleNewtonSolver
+        for k in unused:
+            if k not in ('write_log', 'save_first_and_last_only', 'stop_at_time', 'stop_criteria'):
+                raise TypeError('unknown transient argument ' + k)
+        sel = self._select(select).astype(bool)
+        B = len(self._flamelets)
+        dev = torch.device('cuda:0')
+        lib, h = self._lib, self._h
+        tab = KennedyCarpenterS6P4Q3(SimpleNewtonSolver())
+        A, bw, bh, g = tab.A, tab.b, tab.bh, tab.gamma
+        seg = torch.tensor(np.repeat(np.arange(B), np.diff(self._offsets)), device=dev)
+        w = 1. / torch.tensor(np.concatenate([f._variable_scales for f in self._flamelets]), device=dev)
+        x = torch.tensor(self._x0, device=dev)
+        maxsteps = np.inf if maximum_time_step_count is None else int(maximum_time_step_count)
+        dt = np.broadcast_to(np.asarray(first_time_step, dtype=np.float64), (B,)).copy()
+        t = np.zeros(B)
+        nsteps = np.zeros(B, dtype=np.int64)
+        setup_count = np.zeros(B, dtype=np.int64)
+        eval_setup = np.ones(B, dtype=bool)
+        running = sel.copy()
+        steady = np.zeros(B, dtype=bool)
+        resid = np.full(B, np.nan)
+        stats = {'steps': 0, 'nonlinear_iter': 0, 'projector_setups': 0, 'failed_steps': 0, 'sweeps': 0}
+        mask = lambda m: np.ascontiguousarray(m, dtype=np.int32)
+        on_dofs = lambda m: torch.from_numpy(np.asarray(m, dtype=bool)).to(dev)[seg]
+
+        def fnorm(v):
+            """per flamelet max |v * 1/scale|, NaN where any entry is not finite"""
+            m = torch.zeros(B, dtype=torch.float64, device=dev).scatter_reduce_(0, seg, (v * w).abs(), 'amax')
+            bad = torch.zeros(B, dtype=torch.float64, device=dev).scatter_reduce_(
+                0, seg, (~torch.isfinite(v)).to(torch.float64), 'amax')
+            return torch.where(bad > 0, torch.full_like(m, np.nan), m).cpu().numpy()
+
+        def rhs(m, xv, out):
+            torch.cuda.current_stream().synchronize()
+            check(lib.gf_flamelet_batch_transient_rhs(h, iptr(mask(m)), xv.data_ptr(), out.data_ptr()))
+
+        def setup(m, xv):
+            torch.cuda.current_stream().synchronize()
+            pre = np.ascontiguousarray(dt * g)
+            check(lib.gf_flamelet_batch_transient_setup(h, iptr(mask(m)), xv.data_ptr(),
+                                                        pre.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                        int(lu_precision)))
+            stats['projector_setups'] += int(m.sum())
+
+        def solve(m, r, out):
+            torch.cuda.current_stream().synchronize()
+            check(lib.gf_flamelet_batch_transient_solve(h, iptr(mask(m)), r.data_ptr(), out.data_ptr()))
+
+        d = torch.zeros_like(x)
+        while True:
+            run = running & ~steady
+            if not run.any():
+                break
+            stats['sweeps'] += 1
+            need = run & eval_setup
+            if need.any():
+                setup(need, x)
+            setup_count[run] += 1
+            hd = torch.from_numpy(np.where(run, dt, 0.)).to(dev)[seg]
+            xn = x.clone()
+            f0 = torch.zeros_like(x)
+            rhs(run, xn, f0)
+            ks = [f0]
+            xi = xn.clone()
+            converged = run.copy()
+            for s in range(1, 6):
+                prior = sum(A[s, j] * ks[j] for j in range(s))
+                fi = ks[-1].clone()
+                r = hd * (g * fi + prior) - (xi - xn)
+                done = ~run
+                for it in range(max_nonlinear_iter):
+                    act = ~done
+                    solve(act, r, d)
+                    xi = torch.where(on_dofs(act), xi - d, xi)
+                    rhs(act, xi, fi)
+                    r = hd * (g * fi + prior) - (xi - xn)
+                    stats['nonlinear_iter'] += int(act.sum())
+                    done = done | (act & (fnorm(r) < nonlinear_solve_tolerance))
+                    if done.all():
+                        break
+                converged &= done  # a stage left unconverged also marks the step's solve slow
+                ks.append(fi.clone())
+            dstate = hd * sum(bw[j] * ks[j] for j in range(6))
+            err = fnorm(dstate - hd * sum(bh[j] * ks[j] for j in range(6)))
+            with np.errstate(invalid='ignore', divide='ignore'):
+                res = fnorm(dstate) / dt
+            acc = run & np.isfinite(res)
+            rej = run & ~np.isfinite(res)
+            xnew = x + dstate
+            xnew = torch.where(xnew < 0., torch.zeros_like(xnew), xnew)
+            x = torch.where(on_dofs(acc), xnew, x)
+            recent = dt.copy()
+            t[acc] += dt[acc]
+            nsteps[acc] += 1
+            with np.errstate(divide='ignore', over='ignore'):
+                ramp = np.minimum(1.1, (transient_tolerance / err) ** 0.1333333333)
+            dn = np.where(err < 1.e-16, dt * 1.1, dt * ramp)  # PI control (its history ratio is 1 here)
+            dn = np.minimum(dn, max_time_step)
+            # Jacobian refresh rule (integrator.py:100-130)
+            rate = acc & (setup_count == maximum_steps_per_jacobian)
+            slow = acc & ~rate & ~converged
+            grow = acc & ~rate & ~slow & (dn > recent * 1.05)
+            shrink = acc & ~rate & ~slow & ~grow & (dn < recent * 0.9)
+            setup_count[rate] = 0
+            dn = np.where(slow, dn * 0.8, dn)
+            dt = np.where(acc, dn, dt)
+            eval_setup = np.where(acc, rate | slow | grow | shrink, eval_setup)
+            dt = np.where(rej, dt * 0.8, dt)
+            eval_setup = eval_setup | rej
+            stats['failed_steps'] += int(rej.sum())
+            resid = np.where(run, res, resid)
+            stop_max = run & (nsteps > maxsteps)
+            steady = steady | (run & ~stop_max & (res < steady_tolerance))
+            running = running & ~stop_max
+            if log_rate and stats['sweeps'] % log_rate == 0:
+                left = running & ~steady
+                print('transient sweep %d: %d of %d flamelets stepping, max steps %d, min t %.3e, '
+                      'max steady residual %.3e' % (stats['sweeps'], left.sum(), sel.sum(), nsteps[sel].max(),
+                                                    t[left].min() if left.any() else t[sel].min(),
+                                                    np.nanmax(resid[left]) if left.any() else 0.),
+                      file=sys.stderr, flush=True)
+        stats['steps'] = int(nsteps[sel].max()) if sel.any() else 0
+        fin = torch.zeros_like(x)
+        if sel.any():
+            rhs(sel, x, fin)  # also makes the final states the batch's current states
+        self.residuals = np.where(sel, fnorm(fin), np.nan)
+        self.transient_info = stats
+        self.transient_times = t
+        conv = (sel & steady).astype(np.int32)
+        return self._finish(sel.astype(np.int32), conv, nsteps.astype(np.int32), raise_on_failure,
+                            'steady flamelet time stepping')
+
+    def compute_steady_state(self, tolerance=1.e-6, lu_precision=64, use_psitc=True, newton_args=None,
+                             psitc_args=None, transient_args=None, raise_on_failure=True, log_rate=0,
+                             use_time_stepping=True):
+        """Flamelet.compute_steady_state (flamelet.py:1705-1778) for every flamelet: Newton (38 iterations) from
+        the initial guess; for those it leaves unconverged, pseudo-transient continuation (400 iterations)
+        from the initial guess; for those still unconverged, ESDIRK64 time stepping to steady state from the
+        initial guess with a first step of 1e-2 times psitc's final smallest pseudo-time step (skipped when
+        use_time_stepping is False). Each stage is batched over its flamelets. Returns (states, converged,
+        iterations);
+        method[f] names the stage that converged flamelet f ('newton', 'psitc', 'transient' or None)."""
+        B = len(self._flamelets)
+        self.residuals = np.full(B, np.nan)
+        self.method = np.array([None] * B, dtype=object)
+        self.stage_counters = {}
+        nargs = {'max_iterations': 38, 'log_rate': log_rate}
+        nargs.update(newton_args or {})
+        nargs['raise_on_failure'] = False
+        states, conv, it = self.steady_solve_newton(tolerance, lu_precision, **nargs)
+        res = self.residuals.copy()
+        self.method[conv] = 'newton'
+        self.stage_counters['newton'] = self.counters
+        todo = ~conv
+        mds = np.full(B, 1.e-6)
+        if use_psitc and todo.any():
+            pargs = {'max_iterations': 400, 'log_rate': log_rate}
+            pargs.update(psitc_args or {})
+            pargs['raise_on_failure'] = False
+            _, c2, it2 = self.steady_solve_psitc(tolerance, lu_precision, select=todo, **pargs)
+            mds[todo] = self.min_ds[todo]
+            res[todo] = self.residuals[todo]
+            now = todo & c2
+            self.method[now] = 'psitc'
+            it[todo] = it2[todo]
+            conv = conv | now
+            todo = ~conv
+            self.stage_counters['psitc'] = self.counters
+        if use_time_stepping and todo.any():
+            targs = {'steady_tolerance': tolerance, 'transient_tolerance': 1.e-8, 'max_time_step': 1.e4,
+                     'first_time_step': 1.e-2 * mds, 'maximum_steps_per_jacobian': 10, 'log_rate': log_rate}
+            targs.update(transient_args or {})
+            targs['raise_on_failure'] = False
+            _, c3, st3 = self.integrate_to_steady(lu_precision=lu_precision, select=todo, **targs)
+            res[todo] = self.residuals[todo]
+            now = todo & c3
+            self.method[now] = 'transient'
+            it[todo] = st3[todo]
+            conv = conv | now
+            self.stage_counters['transient'] = dict(self.transient_info)
+        self.residuals = res
+        states = self.split(self.get_state())
         if raise_on_failure and not np.all(conv):
             raise RuntimeError('steady flamelet solve did not converge for %d of %d flamelets'
                                % (int(B - conv.sum()), B))
-        return states, conv.astype(bool), it
+        return states, conv, it

@@ -21,12 +21,32 @@ def _streams(m):
     return m.stream(stp_air=True), m.stream('TPY', (300., 101325., 'H2:1 N2:1'))
 
 
-def _flamelets(m, chis, npts, cls=None):
+def _flamelets(m, chis, npts, cls=None, ics=None):
     from spitfire_amd.flamelet import Flamelet, FlameletSpec
     cls = cls or Flamelet
     air, fuel = _streams(m)
+    ics = ics or ['Burke-Schumann'] * len(chis)  # FlameletSpec's default initial condition
     return [cls(FlameletSpec(mech_spec=m, oxy_stream=air, fuel_stream=fuel, grid_points=n,
-                             stoich_dissipation_rate=c)) for c, n in zip(chis, npts)]
+                             stoich_dissipation_rate=c, initial_condition=ic)) for c, n, ic in zip(chis, npts, ics)]
+
+
+def _scaled(a, b, ns):
+    return float(np.max(np.abs(a - b) / np.tile(np.r_[1e3, np.ones(ns - 1)], a.size // ns)))
+
+
+def _near_steady_guesses(m, chis, npts):
+    """steady states (batched psitc) with a smooth 1% temperature bump: guesses Newton converges from"""
+    from spitfire_amd.flamelet import FlameletBatch
+    ns = m.n_species
+    states, conv, _ = FlameletBatch(_flamelets(m, chis, npts)).steady_solve_psitc(tolerance=1e-9)
+    assert conv.all()
+    out = []
+    for s, n in zip(states, npts):
+        g = s.copy()
+        zi = np.arange(1, n - 1) / (n - 1.)
+        g[::ns] *= 1. + 0.01 * np.sin(np.pi * zi)
+        out.append(g)
+    return states, out
 
 
 @pytest.mark.parametrize('mech', ['mech_c1_data', 'mech_53_data'])
@@ -58,7 +78,7 @@ def test_batch_steady_states_match_oracle(mech, chis, request, oracle_lib):
     npts = [34] * len(chis)
     say('batch steady', mech, chis)
     fb = FlameletBatch(_flamelets(m, chis, npts))
-    states, conv, its = fb.compute_steady_state(tolerance=1e-8)
+    states, conv, its = fb.steady_solve_psitc(tolerance=1e-8)
     say('  gpu iterations', its.tolist(), 'counters', fb.counters, 'residuals', fb.residuals.tolist())
     assert conv.all()
     for f, g in zip(_flamelets(m, chis, npts, OracleFlamelet), states):
@@ -81,9 +101,9 @@ def test_mixed_precision_lu_converges_to_fp64_solution(mech, chis, request):
     ns = m.n_species
     npts = [34] * len(chis)
     say('mixed precision', mech, chis)
-    s64, c64, i64 = FlameletBatch(_flamelets(m, chis, npts)).compute_steady_state(tolerance=1e-8, lu_precision=64)
+    s64, c64, i64 = FlameletBatch(_flamelets(m, chis, npts)).steady_solve_psitc(tolerance=1e-8, lu_precision=64)
     fb = FlameletBatch(_flamelets(m, chis, npts))
-    s32, c32, i32 = fb.compute_steady_state(tolerance=1e-8, lu_precision=32)
+    s32, c32, i32 = fb.steady_solve_psitc(tolerance=1e-8, lu_precision=32)
     say('  iterations fp64', i64.tolist(), 'fp32', i32.tolist(), 'residuals', fb.residuals.tolist())
     assert c64.all() and c32.all()
     for a, b_ in zip(s32, s64):
@@ -91,3 +111,79 @@ def test_mixed_precision_lu_converges_to_fp64_solution(mech, chis, request):
         e = np.max(np.abs(a - b_) / scale)
         say('  max scaled diff %.3e' % e)
         assert e <= 1e-8
+
+
+def test_batch_newton_matches_reference_newton(mech_c1_data, oracle_lib):
+    """The batched Newton method (steady_solve_newton: frozen -J refreshed only after a line search, the
+    reference's compounding line search, no restarts) against the same method on the oracle flamelets from the
+    same guesses. Near-steady guesses: both converge, in the same number of iterations, to states equal to
+    1e-10 scaled. Burke-Schumann guesses: the same outcome (converged or not)."""
+    from spitfire_amd.flamelet import FlameletBatch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from oracle.flamelet_oracle import OracleFlamelet
+    m = ChemicalMechanismSpec(mech_c1_data)
+    ns = m.n_species
+    chis, npts = [1., 10., 100.], [24, 31, 24]
+    _, guesses = _near_steady_guesses(m, chis, npts)
+    ics = guesses + ['Burke-Schumann'] * 3
+    chis, npts = chis * 2, npts * 2
+    say('batch newton', chis, npts)
+    fb = FlameletBatch(_flamelets(m, chis, npts, ics=ics))
+    states, conv, its = fb.steady_solve_newton(tolerance=1e-9, max_iterations=38, raise_on_failure=False)
+    say('  gpu converged', conv.tolist(), 'iterations', its.tolist(), 'residuals', fb.residuals.tolist())
+    for k, f in enumerate(_flamelets(m, chis, npts, OracleFlamelet, ics=ics)):
+        ref, it, ok = f.steady_solve_newton(tolerance=1e-9, max_iterations=38, verbose=False)
+        say('  flamelet %d: oracle converged %s in %s iterations' % (k, ok, it))
+        assert bool(conv[k]) == bool(ok)
+        if k < 3:
+            assert ok and it == its[k]
+            e = _scaled(states[k], ref, ns)
+            say('    max scaled diff %.3e' % e)
+            assert e <= 1e-10
+
+
+def test_batch_compute_steady_state_stages(mech_c1_data, oracle_lib):
+    """compute_steady_state's cascade. With the default stage limits, near-steady guesses converge in the Newton
+    stage and every flamelet converges. With Newton and psitc capped at 2 iterations (no restarts), flamelets
+    from Burke-Schumann guesses go on to the ESDIRK64 stage (per-flamelet steps, batched) and converge there;
+    the oracle runs the same cascade flamelet by flamelet for comparison of the step counts. All end at the
+    oracle's pseudo-transient steady state from the same guess: 1e-8 scaled from the Newton and psitc stages
+    (all solved to a residual of 1e-9), 1e-6 from time stepping stopped at a steadiness rate of 1e-9.
+    Time stepping stops on |dx/scale|/dt, not on the residual: its final scaled residual can be near 1e-5 in the
+    temperature rows (the reference algorithm on the oracle ends the same way on these cases, 5e-6..1.4e-5)
+    while the state is within about 1e-12 of the steady state."""
+    from spitfire_amd.flamelet import FlameletBatch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from oracle.flamelet_oracle import OracleFlamelet
+    m = ChemicalMechanismSpec(mech_c1_data)
+    ns = m.n_species
+    _, guesses = _near_steady_guesses(m, [1., 100.], [24, 24])
+    cases = [
+        ([1., 100., 3., 30.], [24, 24, 24, 28], guesses + ['Burke-Schumann'] * 2, {}, 1e-8),
+        ([3., 30., 300.], [24, 28, 20], ['Burke-Schumann'] * 3,
+         {'newton_args': {'max_iterations': 2}, 'psitc_args': {'max_iterations': 2, 'adaptive_restart': False}},
+         1e-6),
+    ]
+    for chis, npts, ics, limits, bound in cases:
+        say('cascade', chis, npts, limits)
+        fb = FlameletBatch(_flamelets(m, chis, npts, ics=ics))
+        states, conv, its = fb.compute_steady_state(tolerance=1e-9, **limits)
+        say('  methods', fb.method.tolist(), 'steps/iterations', its.tolist(), 'residuals', fb.residuals.tolist(),
+            'transient', fb.stage_counters.get('transient'))
+        assert conv.all()
+        if limits:
+            assert list(fb.method) == ['transient'] * len(chis)
+        else:
+            assert list(fb.method[:2]) == ['newton', 'newton']
+        for k, f in enumerate(_flamelets(m, chis, npts, OracleFlamelet, ics=ics)):
+            ref, it, ok, _ = f.steady_solve_psitc(tolerance=1e-9)
+            assert ok
+            e = _scaled(states[k], ref, ns)
+            say('  flamelet %d (%s): Tmax %.2f oracle %.2f, max scaled diff %.3e'
+                % (k, fb.method[k], states[k][::ns].max(), ref[::ns].max(), e))
+            assert e <= bound
+            if limits:
+                o = _flamelets(m, chis[k:k + 1], npts[k:k + 1], OracleFlamelet, ics=ics[k:k + 1])[0]
+                o.compute_steady_state(tolerance=1e-9, **limits)
+                say('    oracle cascade: %d time steps (batch %d), max scaled diff to the batch %.3e'
+                    % (o.info['steps'], its[k], _scaled(states[k], o.current_interior_state, ns)))

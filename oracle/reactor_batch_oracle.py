@@ -119,3 +119,65 @@ class OracleReactorBatch(ReactorBatch):
                 self._terms(Xs[i].contiguous(), f, J, aux, cpi, hs, None)
                 L += h * self.b[i] * aux[:, 0] * aux[:, 1] * f[:, 0]
         return L
+
+
+def tangent_linear_heat_release_gradient(rb):
+    """dL/dln A [R, nr] (declared reaction order) of a recorded integration by forward sensitivities: the
+    tangent-linear model of the same frozen-step map the discrete adjoint differentiates, carried forward
+    through every step for all reactions at once (the forward-sensitivity reference for the adjoint).
+
+    Per step (h, stage states X_i, clip mask) and reaction r, with J_i the Jacobian and f_p,i = dq_r/dlnA_r *
+    df/dq_r at X_i:
+        dK_0 = J_0 dx_n + f_p,0,
+        (I - h a_ii J_i) dK_i = J_i (dx_n + h sum_{j<i} a_ij dK_j) + f_p,i,     dX_i = dx_n + h sum_{j<=i} a_ij dK_j,
+        dx_{n+1} = dx_n + h sum_i b_i dK_i, zero in clipped components,
+        dL += h sum_i b_i (grad(qdot)_i . dX_i + rho cp (f_p,i)_T),
+    where f_T = -sum_k h_k w_k / (rho cp), f_Yk = w_k / rho and w_k = M_k sum_r N_kr q_r."""
+    import torch
+    R, ns, nr = rb.R, rb.ns, rb.nr
+    A, b = rb.A, rb.b
+    f = torch.empty((R, ns), dtype=torch.float64)
+    J = torch.empty((R, ns * ns), dtype=torch.float64)
+    aux = torch.empty((R, 4), dtype=torch.float64)
+    cpi = torch.empty((R, ns), dtype=torch.float64)
+    hs = torch.empty((R, ns), dtype=torch.float64)
+    qa = torch.empty((R, nr), dtype=torch.float64)
+    N, mw = rb.N, rb.mw
+    dinvm = 1. / mw[:-1] - 1. / mw[-1]
+    eye = torch.eye(ns, dtype=torch.float64)
+    dx = torch.zeros((R, ns, nr), dtype=torch.float64)
+    dL = torch.zeros((R, nr), dtype=torch.float64)
+    for h, Xs, clip in rb.records:
+        dK = []
+        for i in range(6):
+            x = Xs[i].contiguous()
+            rb._terms(x, f, J, aux, cpi, hs, qa)
+            rho, cp, dcp, mmw = aux[:, 0], aux[:, 1], aux[:, 2], aux[:, 3]
+            Jm = J.view(R, ns, ns).transpose(1, 2)  # [reactor, row, column]
+            dfdq = torch.zeros((R, ns, nr), dtype=torch.float64)  # df/dq_r, griffon reaction order
+            MN = mw[:, None] * N  # [species, reaction]
+            dfdq[:, 1:, :] = MN[None, :-1, :] / rho[:, None, None]
+            dfdq[:, 0, :] = -(hs @ MN) / (rho * cp)[:, None]
+            fp = dfdq * qa[:, None, :]
+            base = dx + h[:, None, None] * sum(A[i, j] * dK[j] for j in range(i)) if i else dx
+            rhs = torch.bmm(Jm, base) + fp
+            if i == 0:
+                dKi = rhs
+            else:
+                M = eye[None] - (h * A[i, i])[:, None, None] * Jm
+                dKi = torch.linalg.solve(M, rhs)
+            dXi = base + (h * A[i, i])[:, None, None] * dKi
+            drho = torch.empty((R, ns), dtype=torch.float64)
+            drho[:, 0] = -rho / x[:, 0]
+            drho[:, 1:] = -(rho * mmw)[:, None] * dinvm[None, :]
+            dcpv = torch.empty((R, ns), dtype=torch.float64)
+            dcpv[:, 0] = dcp
+            dcpv[:, 1:] = cpi[:, :-1] - cpi[:, -1:]
+            gq = (rho * cp)[:, None] * Jm[:, 0, :] + f[:, 0:1] * (cp[:, None] * drho + rho[:, None] * dcpv)
+            dL += (h * b[i])[:, None] * (torch.einsum('rk,rkp->rp', gq, dXi) + (rho * cp)[:, None] * fp[:, 0, :])
+            dK.append(dKi)
+        dx = dx + h[:, None, None] * sum(b[j] * dK[j] for j in range(6))
+        dx = torch.where(clip[:, :, None], torch.zeros_like(dx), dx)
+    out = torch.empty_like(dL)
+    out[:, torch.as_tensor(rb.file_order)] = dL
+    return out

@@ -23,6 +23,9 @@ with I - dt a_ii J_i^T). A clipped component has zero derivative through the cli
 df/dp = (df/dq_r) dq_r/dln A_r, with dq_r/dln A_r from the chemistry kernel (the rate of progress itself for
 elementary and three-body reactions; for falloff reactions A is the high-pressure factor and the falloff
 function's dependence on it is included).
+
+heat_release(log_A, ...) wraps forward integration and adjoint as a torch.autograd.Function of ln A; the stage
+states of every step are the checkpoints the backward pass runs over.
 """
 import numpy as np
 
@@ -294,3 +297,72 @@ class ReactorBatch:
         out[:, torch.as_tensor(self.file_order, device=dev)] = grad
         self.lambda0 = lam
         return L, out
+
+
+def _pre_exponential_key(rx):
+    return 'A' if 'A' in rx else 'fwd-A'  # falloff reactions: the high-pressure (forward) A
+
+
+def log_pre_exponentials(mech_spec):
+    """ln A of every reaction in the mechanism's declared order, in the units of its mechanism data: the A of
+    the forward rate, the high-pressure A for falloff reactions (the parameters heat_release_gradient
+    differentiates against)."""
+    return np.log([float(rx[_pre_exponential_key(rx)]) for rx in mech_spec.mech_data['reactions']])
+
+
+def with_log_pre_exponentials(mech_spec, log_A):
+    """A copy of the mechanism whose reactions (declared order) have the pre-exponential factors exp(log_A)"""
+    import copy
+    from .mechanism import ChemicalMechanismSpec
+    log_A = np.asarray(log_A, dtype=np.float64)
+    d = copy.deepcopy(mech_spec.mech_data)
+    if log_A.shape != (len(d['reactions']),):
+        raise ValueError('log_A needs one value per reaction (%d)' % len(d['reactions']))
+    for rx, la in zip(d['reactions'], log_A):
+        rx[_pre_exponential_key(rx)] = float(np.exp(la))
+    return ChemicalMechanismSpec(d)
+
+
+def _heat_release_function():
+    import torch
+
+    class HeatRelease(torch.autograd.Function):
+        """L = integrated heat release of every reactor of a batch, as a differentiable function of ln A.
+
+        forward(log_A, mech_spec, T0, Y0, pressure, t_end, integrate_kwargs) integrates the reactors with the
+        mechanism's pre-exponential factors replaced by exp(log_A) (declared order; the mechanism is rebuilt
+        only when log_A differs from its own values), recording every stage state, and returns L [R]. backward
+        runs the discrete adjoint over the recorded stages (ReactorBatch.heat_release_gradient) and returns
+        sum_r dLoss/dL_r dL_r/dlog_A."""
+
+        @staticmethod
+        def forward(ctx, log_A, mech_spec, T0, Y0, pressure, t_end, integrate_kwargs):
+            la = log_A.detach().cpu().numpy().astype(np.float64)
+            own = log_pre_exponentials(mech_spec)
+            spec = mech_spec if np.array_equal(la, own) else with_log_pre_exponentials(mech_spec, la)
+            rb = ReactorBatch(spec, T0, Y0, pressure)
+            rb.integrate(t_end, record=True, **(integrate_kwargs or {}))
+            ctx.batch = rb
+            ctx.device = log_A.device
+            return rb.heat_release().to(log_A.device)
+
+        @staticmethod
+        def backward(ctx, grad_L):
+            _, G = ctx.batch.heat_release_gradient()
+            g = (grad_L.to(G.device, torch.float64)[:, None] * G).sum(dim=0)
+            return g.to(ctx.device), None, None, None, None, None, None
+
+    return HeatRelease
+
+
+_HEAT_RELEASE = None
+
+
+def heat_release(log_A, mech_spec, T0, Y0, pressure=101325., t_end=2.e-3, **integrate_kwargs):
+    """Differentiable integrated heat release [R] of reactors started at (T0 [R], Y0 [R, ns]) with
+    pre-exponential factors exp(log_A) (a torch tensor, one per reaction in declared order): the forward
+    integration of ReactorBatch, with the discrete adjoint as the backward pass (torch.autograd.Function)."""
+    global _HEAT_RELEASE
+    if _HEAT_RELEASE is None:
+        _HEAT_RELEASE = _heat_release_function()
+    return _HEAT_RELEASE.apply(log_A, mech_spec, T0, Y0, pressure, t_end, integrate_kwargs)

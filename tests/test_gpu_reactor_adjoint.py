@@ -132,3 +132,39 @@ def test_gpu_adjoint_matches_gpu_differences_53(mech_53_data):
     err = np.max(np.abs(G[pick] - fd)) / np.max(np.abs(fd))
     say('steps %d, reactions %s, normwise error %.3e' % (g.stats['steps'], pick, err))
     assert err <= 1e-4
+
+
+def test_heat_release_autograd_function(mech_c1_data):
+    """reactor_batch.heat_release as a torch.autograd.Function: its forward value equals ReactorBatch's
+    heat_release() of the same integration, and backward of a weighted sum over the reactors equals the same
+    weights applied to the discrete adjoint (heat_release_gradient) of that integration, to 1e-12 normwise
+    (the integrations are the same deterministic kernels). A shifted ln A rebuilds the mechanism: the loss
+    changes, and its backward is the adjoint of the shifted mechanism."""
+    import torch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from spitfire_amd.reactor_batch import ReactorBatch, heat_release, log_pre_exponentials, \
+        with_log_pre_exponentials
+    m = ChemicalMechanismSpec(mech_c1_data)
+    T0, Y0 = c1_reactors(m, [1200., 1500., 1350.], [1.0, 0.6, 1.6])
+    dev = torch.device('cuda:0')
+    wts = torch.tensor([0.3, -1.7, 1.1], dtype=torch.float64, device=dev)
+    la0 = log_pre_exponentials(m)
+    shift = np.zeros_like(la0)
+    shift[[1, 5]] = [0.2, -0.3]
+    for la_np in (la0, la0 + shift):
+        say('autograd heat release, ln A shifted by', shift if la_np is not la0 else 0)
+        la = torch.tensor(la_np, device=dev, requires_grad=True)
+        L = heat_release(la, m, T0, Y0, t_end=1.e-4)
+        (L * wts).sum().backward()
+        rb = ReactorBatch(m if la_np is la0 else with_log_pre_exponentials(m, la_np), T0, Y0)
+        rb.integrate(1.e-4)
+        L2, G = rb.heat_release_gradient()
+        ref = (wts[:, None] * G).sum(dim=0)
+        eL = float((L - L2).abs().max() / L2.abs().max())
+        eg = float((la.grad - ref).abs().max() / ref.abs().max())
+        say('  L', L.tolist(), 'rel diff %.3e, gradient normwise diff %.3e' % (eL, eg))
+        assert eL <= 1e-12 and eg <= 1e-12
+        if la_np is la0:
+            L0 = L.detach().clone()
+        else:
+            assert float((L.detach() - L0).abs().max()) > 1e-6 * float(L0.abs().max())

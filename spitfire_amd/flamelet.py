@@ -509,11 +509,13 @@ class Flamelet:
 class FlameletBatch:
     """Many adiabatic flamelets of one mechanism and pressure solved together on the GPU.
 
-    The flamelets keep their own grids, dissipation rates, boundary streams and initial conditions; the
-    steady solve is the reference's pseudo-transient continuation (Flamelet.steady_solve_psitc,
-    flamelet.py:1487-1703) applied to every flamelet with device-resident state and batched kernels
-    (gf_flamelet_batch_* in include/griffon_amd.h). Converged states are written back to each Flamelet
-    (current_interior_state), as compute_steady_state does for one flamelet.
+    The flamelets keep their own grids, dissipation rates, boundary streams and initial conditions. The steady
+    solvers are the reference's, applied to every flamelet with device-resident state and batched kernels
+    (gf_flamelet_batch_* in include/griffon_amd.h): Newton (Flamelet.steady_solve_newton,
+    flamelet.py:1352-1485), pseudo-transient continuation (steady_solve_psitc, flamelet.py:1487-1703),
+    ESDIRK64 time stepping to steady state (integrate_to_steady, flamelet.py:1215-1237), and the cascade of
+    the three (compute_steady_state, flamelet.py:1705-1778). Converged states are written back to each
+    Flamelet (current_interior_state), as compute_steady_state does for one flamelet.
     """
 
     def __init__(self, flamelets):

@@ -5,8 +5,9 @@ Forward: the step loop of HomogeneousReactor.integrate with its defaults (reacto
 methods.py:441-612; frozen-Jacobian Newton, nonlinear.py:185-268, tolerance 1e-12 on the scaled residual;
 PI control, stepcontrol.py:50-117; a Jacobian setup every step; negative entries clipped after every step),
 run for all reactors in lock step. Each reactor keeps its own time, step size, Newton convergence and clipping.
-The chemistry (RHS, Jacobian) runs in the batched HIP kernel; every dt*gamma*J - I is factored and solved by the
-batched block LU with one block per system.
+The chemistry (RHS, Jacobian) runs in the batched HIP kernel; every dt*gamma*J - I is inverted by the batched
+register-resident Gauss-Jordan kernel (partial pivoting, one block per system) and each Newton solve is a
+matrix-vector product with that inverse.
 
 Loss: L = integral over [0, t_end] of qdot = -sum_k h_k w_k (= rho cp dT/dt for a closed adiabatic isobaric
 reactor, isobaric_reactor_kernels.cpp:23), discretised by the Runge-Kutta weights of each step, i.e. as an
@@ -33,8 +34,9 @@ from .time_integration import KennedyCarpenterS6P4Q3, SimpleNewtonSolver
 
 
 class _BatchedLU:
-    """Dense LU with partial pivoting of R independent ns x ns column-major blocks (the block-tridiagonal
-    kernels with one block per system)."""
+    """Solves with R independent ns x ns column-major blocks: each block inverted once per factor() by the
+    register-resident Gauss-Jordan kernel with partial pivoting (the one-block case of the batched
+    block-tridiagonal inverse form), each solve one matrix-vector product with the stored FP64 inverse."""
 
     def __init__(self, R, ns, device):
         import torch
@@ -45,24 +47,24 @@ class _BatchedLU:
         i64 = dict(dtype=torch.int64, device=device)
         self.nb = torch.ones(R, dtype=torch.int32, device=device)
         self.voff = torch.arange(R, **i64) * ns * ns
-        self.poff = torch.arange(R, **i64) * ns
-        self.L = torch.zeros(R * ns * ns, dtype=torch.float64, device=device)
-        self.piv = torch.zeros(R * ns, dtype=torch.int32, device=device)
+        self.xoff = torch.arange(R, **i64) * ns
+        self.inv = torch.zeros(R * ns * ns, dtype=torch.float64, device=device)
         self.values = None
 
     def factor(self, values, stream):
-        """values: contiguous [R, ns*ns] column-major blocks, factored in place"""
+        """values: contiguous [R, ns*ns] column-major blocks (left unmodified)"""
         self.values = values
-        self._check(self._lib.gf_batch_btddod_full_factorize(self.R, self.nb.data_ptr(), self.ns, values.data_ptr(),
-                                                             self.voff.data_ptr(), self.L.data_ptr(),
-                                                             self.voff.data_ptr(), self.piv.data_ptr(),
-                                                             self.poff.data_ptr(), stream))
+        self._check(self._lib.gf_batch_btddod_inverse_factorize(self.R, self.nb.data_ptr(), self.ns,
+                                                                values.data_ptr(), self.voff.data_ptr(),
+                                                                self.inv.data_ptr(), self.voff.data_ptr(), 64,
+                                                                stream))
 
     def solve(self, rhs, out, stream):
-        self._check(self._lib.gf_batch_btddod_full_solve(self.R, self.nb.data_ptr(), self.ns, self.values.data_ptr(),
-                                                         self.voff.data_ptr(), self.L.data_ptr(), self.voff.data_ptr(),
-                                                         self.piv.data_ptr(), self.poff.data_ptr(), rhs.data_ptr(),
-                                                         out.data_ptr(), self.poff.data_ptr(), stream))
+        self._check(self._lib.gf_batch_btddod_inverse_solve(self.R, self.nb.data_ptr(), self.ns,
+                                                            self.values.data_ptr(), self.voff.data_ptr(),
+                                                            self.inv.data_ptr(), self.voff.data_ptr(), 64,
+                                                            rhs.data_ptr(), out.data_ptr(), self.xoff.data_ptr(),
+                                                            stream))
         return out
 
 

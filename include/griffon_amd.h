@@ -100,6 +100,34 @@ int gf_btddod_scale_and_add_diagonal(double *inout_values, double matrix_scale, 
                                      double diag_scale, int num_blocks, int block_size);          /* griffon.pyx:1107-1113 */
 
 /* ---------------- batched device entry points (device pointers, stream-ordered) ---------------------- */
+/* Vector steps of batched implicit stage solves (no counterpart in the reference; they fuse the elementwise
+ * work of the frozen-Jacobian Newton method, nonlinear.py:185-268, for many independent systems of n
+ * unknowns, system s at elements s*n .. s*n+n-1; device pointers; conv is one byte per system, 0 or 1).
+ *   masked_update:  x -= d for the systems with conv[s] == 0
+ *   stage_residual: r = h[s] (gamma f + prior) - (x - xn); with test != 0 also conv[s] = 1 where
+ *                   max_v |r_v w_v| < tol (w: n weights). Same operations, same order, no contraction. */
+int gf_batch_masked_update(int R, int n, double *x, const double *d, const unsigned char *conv, void *stream);
+int gf_batch_stage_residual(int R, int n, const double *h, double gamma, const double *f, const double *prior,
+                            const double *x, const double *xn, const double *w, double tol, double *r,
+                            unsigned char *conv, int test, void *stream);
+/* One implicit stage of the discrete adjoint of batched closed adiabatic isobaric reactor integrations (no
+ * counterpart in the reference; device pointers, R reactors of n = ns state variables, 2 <= n <= 64).
+ * x, f, jac, aux, cpi: the stage state, RHS, column-major Jacobian, [rho, cp, dcp/dT, mixture molecular
+ * weight] and species cp, as gf_batch_reactor_jac_isobaric_terms gives them; dinvm[k] = 1/M_k - 1/M_{ns-1}.
+ *   stage_prep:   rhs = h (J^T v + b_i grad(qdot)), qdot = rho cp f_T; m (may be null) = I - h a_ii J^T
+ *   stage_finish: wv = v + a_ii theta, wT = wv_0 + b_i rho cp, C_k = wv_{k+1}/rho (0 for the last),
+ *                 C_k -= wT/(rho cp) h_k, cm_k = C_k M_k; L += h b_i rho cp f_T */
+int gf_batch_adjoint_stage_prep(int R, int n, const double *x, const double *f, const double *jac, const double *aux,
+                                const double *cpi, const double *dinvm, const double *v, const double *h, double b_i,
+                                double a_ii, double *rhs, double *m, void *stream);
+int gf_batch_adjoint_stage_finish(int R, int n, const double *v, const double *theta, const double *aux,
+                                  const double *hsp, const double *mw, const double *f, const double *h, double b_i,
+                                  double a_ii, double *cm, double *L, void *stream);
+/* Launch shape of the batched chemistry kernels (no counterpart in the reference): -1 (the default) times the
+ * feasible shapes at the first launch of each kind (RHS + Jacobian, RHS only) and batch-size class
+ * [2^k, 2^(k+1)) on a mechanism and keeps the fastest; 0..4 fixes 1x128, 2x256, 4x512, 4x256 or 2x512
+ * (states x threads per workgroup). All shapes give bitwise identical results. Process-wide. */
+int gf_set_chem_launch_shape(int shape);
 /* Closed adiabatic isobaric reactor chemistry over n independent states. Element v of state i is read
  * from x[i*x_item_stride + v*x_var_stride] (x_item_stride=1, x_var_stride=n is structure-of-arrays);
  * p[i*p_stride] is the pressure (p_stride 0 = one pressure). rhs element v of state i goes to

@@ -1,6 +1,8 @@
 """TEST INFRASTRUCTURE ONLY — spitfire_amd.reactor_batch.ReactorBatch with its chemistry and linear algebra on
 the CPU oracle (oracle/griffon_oracle.cpp, SciPy LU) instead of the GPU: the checker for the batched reactor
-integration and its discrete adjoint. The step loop and the adjoint recursion are the product's, unchanged.
+integration and its discrete adjoint. The step loop and the adjoint recursion are the product's; the two fused
+Newton-step kernels are restated here elementwise (x -= d where unconverged; the stage residual and its
+convergence test).
 
 The oracle has no per-reaction rate-of-progress output, so d(q_r)/d(ln A_r) is taken here from central
 differences of the oracle's production rates over mechanisms with A_r scaled by exp(+-eps):
@@ -51,6 +53,48 @@ class OracleReactorBatch(ReactorBatch):
 
     def _make_lu(self):
         return _CpuLU(self.R, self.ns)
+
+    def _adjoint_prep(self, x, f, J, aux, cpi, dinvm, v, h, bi, aii, rhs, M):
+        """rhs = h (J^T v + b_i grad qdot), grad qdot = rho cp dfT/dx + f_T (cp drho/dx + rho dcp/dx); M = I - h a_ii
+        J^T (column-major), elementwise"""
+        import torch
+        R, ns = self.R, self.ns
+        rho, cp, dcp, mmw = aux[:, 0], aux[:, 1], aux[:, 2], aux[:, 3]
+        Jc = J.view(R, ns, ns)  # Jc[r, column, row]
+        fT = f[:, 0]
+        drho = torch.empty((R, ns), dtype=torch.float64)
+        drho[:, 0] = -rho / x[:, 0]
+        drho[:, 1:] = -(rho * mmw)[:, None] * dinvm[None, :]
+        dcpv = torch.empty((R, ns), dtype=torch.float64)
+        dcpv[:, 0] = dcp
+        dcpv[:, 1:] = cpi[:, :-1] - cpi[:, -1:]
+        gq = (rho * cp)[:, None] * Jc[:, :, 0] + fT[:, None] * (cp[:, None] * drho + rho[:, None] * dcpv)
+        rhs.copy_(h[:, None] * (torch.bmm(Jc, v[:, :, None])[:, :, 0] + bi * gq))
+        if M is not None:
+            eye = torch.eye(ns, dtype=torch.float64)
+            M.copy_((eye - (h * aii)[:, None, None] * Jc.transpose(1, 2)).reshape(R, ns * ns))
+
+    def _adjoint_finish(self, v, theta, aux, hs, f, h, bi, aii, cm, L):
+        """wv = v + a_ii theta; C from wv and the species enthalpies; cm = C M; L += h b_i rho cp f_T"""
+        import torch
+        rho, cp = aux[:, 0], aux[:, 1]
+        wv = v + aii * theta
+        wT = wv[:, 0] + bi * rho * cp
+        C = torch.zeros((self.R, self.ns), dtype=torch.float64)
+        C[:, :-1] = wv[:, 1:] / rho[:, None]
+        C = C - (wT / (rho * cp))[:, None] * hs
+        cm.copy_(C * self.mw[None, :])
+        L += h * bi * rho * cp * f[:, 0]
+
+    def _masked_update(self, x, d, conv):
+        import torch
+        x.copy_(torch.where(conv[:, None], x, x - d))
+
+    def _stage_residual(self, h, g, f, prior, x, xn, r, conv, test, tol):
+        import torch
+        r.copy_(h[:, None] * (g * f + prior) - (x - xn))
+        if test:
+            conv |= torch.max(torch.abs(r * self.w), dim=1).values < tol
 
     def _soa(self, x):
         return np.ascontiguousarray(x.numpy().T)

@@ -46,6 +46,11 @@ SIGNATURES = {
     'gf_btddod_full_factorize': [_D, _int, _int, _D, _I],
     'gf_btddod_full_solve': [_D, _D, _I, _D, _int, _int, _D],
     'gf_btddod_scale_and_add_diagonal': [_D, _dbl, _D, _dbl, _int, _int],
+    'gf_set_chem_launch_shape': [_int],
+    'gf_batch_masked_update': [_int, _int, _V, _V, _V, _V],
+    'gf_batch_adjoint_stage_prep': [_int, _int, _V, _V, _V, _V, _V, _V, _V, _V, _dbl, _dbl, _V, _V, _V],
+    'gf_batch_adjoint_stage_finish': [_int, _int, _V, _V, _V, _V, _V, _V, _V, _dbl, _dbl, _V, _V, _V],
+    'gf_batch_stage_residual': [_int, _int, _V, _dbl, _V, _V, _V, _V, _V, _dbl, _V, _V, _int, _V],
     'gf_batch_reactor_rhs_isobaric': [_V, _int, _V, _long, _long, _V, _long, _V, _long, _long, _V],
     'gf_batch_reactor_jac_isobaric': [_V, _int, _V, _long, _long, _V, _long, _V, _long, _long, _V, _long, _V],
     'gf_batch_btddod_full_factorize': [_int, _V, _int, _V, _V, _V, _V, _V, _V, _V],

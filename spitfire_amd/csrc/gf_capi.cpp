@@ -379,6 +379,8 @@ int gf_btddod_scale_and_add_diagonal(double *values, double a, const double *dia
 
 // ---------------- batched device entry points ----------------
 
+int gf_set_chem_launch_shape(int shape) { return set_chem_launch_shape(shape); }
+
 int gf_batch_reactor_rhs_isobaric(gf_mech *m, int n, const double *x, long xs_i, long xs_v, const double *p, long ps,
                                   double *rhs, long rs_i, long rs_v, void *stream) {
   if (!m) return fail(GF_ERR_ARG, "null mechanism");

@@ -635,28 +635,22 @@ int launch_cfg(const gf_mech *m, const ChemArgs &a, hipStream_t st) {
 
 }  // namespace
 
-// Launch configuration: states per workgroup S and threads per workgroup. GF_CHEM_CFG=<S>x<threads>
-// overrides the default (used by the launch-shape sweep); configurations whose LDS does not fit the
-// device fall back to fewer states per workgroup.
-int launch_chem(const gf_mech *m, const ChemIO &a, hipStream_t st) {
-  if (a.n <= 0) return GF_OK;
-  static int cfg = -1;
-  if (cfg < 0) {
-    cfg = 2;
-    if (const char *e = getenv("GF_CHEM_CFG")) {
-      if (!strcmp(e, "1x128")) cfg = 0;
-      else if (!strcmp(e, "2x256")) cfg = 1;
-      else if (!strcmp(e, "4x512")) cfg = 2;
-      else if (!strcmp(e, "4x256")) cfg = 3;
-      else if (!strcmp(e, "2x512")) cfg = 4;
-    }
-  }
-  const bool J = a.jac || a.sens;
-  int c = cfg;
+// Launch shapes: states per workgroup x threads per workgroup. Every shape computes each state with the same
+// lanes in the same order (per-state waves and reductions do not depend on S), so their outputs are bitwise
+// identical and only their speed differs.
+namespace {
+constexpr int kShapes = 5;  // 0: 1x128, 1: 2x256, 2: 4x512, 3: 4x256, 4: 2x512
+
+bool shape_fits(const gf_mech *m, int c, bool J) {
+  if (!J) return true;
+  const size_t lim = m->lds_max;
+  if (c == 2 || c == 3) return lds_bytes<4, 512, true>(m) <= lim;
+  if (c == 1 || c == 4) return lds_bytes<2, 256, true>(m) <= lim;
+  return true;
+}
+
+int launch_shape(const gf_mech *m, const ChemIO &a, hipStream_t st, int c, bool J) {
   if (J) {
-    const size_t lim = m->lds_max;
-    if ((c == 2 || c == 3) && lds_bytes<4, 512, true>(m) > lim) c = 1;
-    if ((c == 1 || c == 4) && lds_bytes<2, 256, true>(m) > lim) c = 0;
     switch (c) {
       case 0: return launch_cfg<1, 128, true>(m, a, st);
       case 1: return launch_cfg<2, 256, true>(m, a, st);
@@ -668,10 +662,100 @@ int launch_chem(const gf_mech *m, const ChemIO &a, hipStream_t st) {
   switch (c) {
     case 0: return launch_cfg<1, 128, false>(m, a, st);
     case 1: return launch_cfg<2, 256, false>(m, a, st);
+    case 3: return launch_cfg<4, 256, false>(m, a, st);
     case 4: return launch_cfg<2, 512, false>(m, a, st);
-    case 2: return launch_cfg<4, 512, false>(m, a, st);
-    default: return launch_cfg<4, 256, false>(m, a, st);
+    default: return launch_cfg<4, 512, false>(m, a, st);
   }
+}
+
+// The fastest feasible shape for this launch: each candidate runs once to warm up, then twice between HIP
+// events on the caller's stream (the outputs are the caller's, written identically by every candidate).
+int autotune_shape(const gf_mech *m, const ChemIO &a, hipStream_t st, bool J, int &best) {
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return fail(GF_ERR_HIP, "hipEventCreate failed");
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(GF_ERR_HIP, "hipEventCreate failed");
+  }
+  float tbest = 0.f;
+  bool have = false;
+  best = 2;
+  int rc = GF_OK;
+  for (int c = 0; c < kShapes && rc == GF_OK; ++c) {
+    if (!shape_fits(m, c, J)) continue;
+    rc = launch_shape(m, a, st, c, J);
+    if (rc == GF_OK && hipEventRecord(e0, st) != hipSuccess) rc = fail(GF_ERR_HIP, "hipEventRecord failed");
+    for (int k = 0; k < 2 && rc == GF_OK; ++k) rc = launch_shape(m, a, st, c, J);
+    if (rc == GF_OK && hipEventRecord(e1, st) != hipSuccess) rc = fail(GF_ERR_HIP, "hipEventRecord failed");
+    if (rc == GF_OK && hipEventSynchronize(e1) != hipSuccess) rc = fail(GF_ERR_HIP, "hipEventSynchronize failed");
+    float ms = 0.f;
+    if (rc == GF_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(GF_ERR_HIP, "hipEventElapsedTime failed");
+    if (rc == GF_OK && (!have || ms < tbest)) {
+      have = true;
+      tbest = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+}  // namespace
+
+// GF_CHEM_CFG=<S>x<threads> (or gf_set_chem_launch_shape) fixes the launch shape for every call. Otherwise
+// the first launch of a kind
+// (RHS + Jacobian, or RHS only) at a batch size in [2^k, 2^(k+1)) times the feasible shapes and the fastest is
+// kept for that kind and size class on this mechanism; under stream capture the default 4x512 is used
+// untimed. Concurrent first calls from several host threads on one mechanism may time twice.
+namespace {
+int g_fixed_shape = -2;  // -2: not read from the environment yet; -1: autotune; 0..4: that shape
+int fixed_shape() {
+  if (g_fixed_shape == -2) {
+    g_fixed_shape = -1;
+    if (const char *e = getenv("GF_CHEM_CFG")) {
+      if (!strcmp(e, "1x128")) g_fixed_shape = 0;
+      else if (!strcmp(e, "2x256")) g_fixed_shape = 1;
+      else if (!strcmp(e, "4x512")) g_fixed_shape = 2;
+      else if (!strcmp(e, "4x256")) g_fixed_shape = 3;
+      else if (!strcmp(e, "2x512")) g_fixed_shape = 4;
+    }
+  }
+  return g_fixed_shape;
+}
+}  // namespace
+
+int set_chem_launch_shape(int shape) {
+  if (shape < -1 || shape >= kShapes) return fail(GF_ERR_ARG, "launch shape must be -1 (timed) or 0..4");
+  g_fixed_shape = shape;
+  return GF_OK;
+}
+
+int launch_chem(const gf_mech *m, const ChemIO &a, hipStream_t st) {
+  if (a.n <= 0) return GF_OK;
+  const int fixed = fixed_shape();
+  const bool J = a.jac || a.sens;
+  auto fallback = [&](int c) {  // fewer states per workgroup where the LDS does not fit
+    if ((c == 2 || c == 3) && !shape_fits(m, c, J)) c = 1;
+    if ((c == 1 || c == 4) && !shape_fits(m, c, J)) c = 0;
+    return c;
+  };
+  if (fixed >= 0) return launch_shape(m, a, st, fallback(fixed), J);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return launch_shape(m, a, st, fallback(2), J);
+  if (!m->chem_shape_init) {
+    std::memset(m->chem_shape, -1, sizeof m->chem_shape);
+    m->chem_shape_init = true;
+  }
+  const int k = 63 - __builtin_clzll((unsigned long long)a.n);
+  signed char &slot = m->chem_shape[J ? 0 : 1][k];
+  if (slot < 0) {
+    int best = 2;
+    const int rc = autotune_shape(m, a, st, J, best);
+    if (rc != GF_OK) return rc;
+    slot = (signed char)best;
+  }
+  return launch_shape(m, a, st, slot, J);
 }
 
 }  // namespace gfa

@@ -112,6 +112,10 @@ struct gf_mech {
   size_t lds_per_state_rhs = 0;   // the same for the RHS-only kernel
   int n_cu = 0;                   // compute units of the device (persistent launch width)
   size_t lds_max = 65536;         // largest LDS allocation one workgroup may use on this device
+  // chemistry launch shape picked by timing, per [Jacobian or RHS-only kernel][floor(log2(states))]; -1: not
+  // yet timed (launch_chem, gf_chem.hip)
+  mutable signed char chem_shape[2][64] = {};
+  mutable bool chem_shape_init = false;
   // staging buffers for the host-pointer drop-ins
   void *stage = nullptr;
   size_t stage_bytes = 0;
@@ -148,6 +152,7 @@ struct ChemIO {
   // optional: the Jacobian block of physical row q starts at jac + js_off[q] (no block if negative)
   const long *js_off = nullptr;
 };
+int set_chem_launch_shape(int shape);  // -1: timed per launch kind and size class; 0..4: fixed
 int launch_chem(const gf_mech *m, const ChemIO &io, hipStream_t st);
 
 // flamelet kernels (gf_flamelet.hip)

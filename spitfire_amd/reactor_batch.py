@@ -121,6 +121,31 @@ class ReactorBatch:
     def _jac(self, x, rhs, jac):
         self._g.batch_reactor_jac_isobaric(x, self.p, rhs, jac, layout='aos')
 
+    def _adjoint_prep(self, x, f, J, aux, cpi, dinvm, v, h, bi, aii, rhs, M):
+        from . import _native
+        _native.check(_native.lib().gf_batch_adjoint_stage_prep(
+            self.R, self.ns, x.data_ptr(), f.data_ptr(), J.data_ptr(), aux.data_ptr(), cpi.data_ptr(),
+            dinvm.data_ptr(), v.data_ptr(), h.data_ptr(), float(bi), float(aii), rhs.data_ptr(),
+            None if M is None else M.data_ptr(), self._stream()))
+
+    def _adjoint_finish(self, v, theta, aux, hs, f, h, bi, aii, cm, L):
+        from . import _native
+        _native.check(_native.lib().gf_batch_adjoint_stage_finish(
+            self.R, self.ns, v.data_ptr(), theta.data_ptr(), aux.data_ptr(), hs.data_ptr(), self.mw.data_ptr(),
+            f.data_ptr(), h.data_ptr(), float(bi), float(aii), cm.data_ptr(), L.data_ptr(), self._stream()))
+
+    def _masked_update(self, x, d, conv):
+        from . import _native
+        _native.check(_native.lib().gf_batch_masked_update(self.R, self.ns, x.data_ptr(), d.data_ptr(),
+                                                           conv.data_ptr(), self._stream()))
+
+    def _stage_residual(self, h, g, f, prior, x, xn, r, conv, test, tol):
+        from . import _native
+        _native.check(_native.lib().gf_batch_stage_residual(self.R, self.ns, h.data_ptr(), float(g), f.data_ptr(),
+                                                            prior.data_ptr(), x.data_ptr(), xn.data_ptr(),
+                                                            self.w.data_ptr(), float(tol), r.data_ptr(),
+                                                            conv.data_ptr(), int(test), self._stream()))
+
     # ---------------- forward ----------------
     def integrate(self, t_end, first_time_step=1.e-6, max_time_step=1.e6, transient_tolerance=1.e-10,
                   nonlinear_solve_tolerance=1.e-12, max_nonlinear_iter=20, dt_schedule=None, record=True,
@@ -162,17 +187,19 @@ class ReactorBatch:
             X = [xn.clone()]
             xs = xn.clone()
             for s in range(1, 6):
-                prior = sum(A[s, j] * ks[j] for j in range(s))
+                prior = sum(A[s, j] * ks[j] for j in range(s)).contiguous()
                 xi = xs.clone()
                 fi = ks[-1].clone()
-                r = h[:, None] * (g * fi + prior) - (xi - xn)
+                r = torch.empty_like(xi)
                 conv = torch.zeros(R, dtype=torch.bool, device=self.dev)
+                # stage residual r = h (gamma f + prior) - (x - xn), then Newton: x -= solve(r) for the
+                # reactors not yet converged, f(x), residual and convergence test (fused kernels)
+                self._stage_residual(h, g, fi, prior, xi, xn, r, conv, False, nonlinear_solve_tolerance)
                 for it in range(max_nonlinear_iter):
                     self.lu.solve(r, d, st)
-                    xi = torch.where(conv[:, None], xi, xi - d)
+                    self._masked_update(xi, d, conv)
                     self._rhs(xi, fi)
-                    r = h[:, None] * (g * fi + prior) - (xi - xn)
-                    conv = conv | (torch.max(torch.abs(r * w), dim=1).values < nonlinear_solve_tolerance)
+                    self._stage_residual(h, g, fi, prior, xi, xn, r, conv, True, nonlinear_solve_tolerance)
                     self.stats['newton_iterations'] += 1
                     if bool(conv.all()):
                         break
@@ -248,13 +275,15 @@ class ReactorBatch:
         cpi = torch.empty((R, ns), dtype=torch.float64, device=dev)
         hs = torch.empty((R, ns), dtype=torch.float64, device=dev)
         qa = torch.empty((R, nr), dtype=torch.float64, device=dev)
-        eye = torch.eye(ns, dtype=torch.float64, device=dev)
         dinvm = (1. / self.mw[:-1] - 1. / self.mw[-1])
         lam = torch.zeros((R, ns), dtype=torch.float64, device=dev)
         grad = torch.zeros((R, nr), dtype=torch.float64, device=dev)
         L = torch.zeros(R, dtype=torch.float64, device=dev)
         st = self._stream()
         th = torch.empty((R, ns), dtype=torch.float64, device=dev)
+        Mt = torch.empty((R, ns * ns), dtype=torch.float64, device=dev)
+        rhs = torch.empty((R, ns), dtype=torch.float64, device=dev)
+        cm = torch.empty((R, ns), dtype=torch.float64, device=dev)
         for n_back, (h, Xs, clip) in enumerate(reversed(self.records)):
             if log_rate and n_back % log_rate == 0:
                 import sys
@@ -264,36 +293,21 @@ class ReactorBatch:
             for i in range(5, -1, -1):
                 x = Xs[i].contiguous()
                 self._terms(x, f, J, aux, cpi, hs, qa)
-                rho, cp, dcp, mmw = aux[:, 0], aux[:, 1], aux[:, 2], aux[:, 3]
-                Jc = J.view(R, ns, ns)  # Jc[r, column, row]
-                fT = f[:, 0]
-                # gradient of qdot = rho cp dT/dt with respect to (T, Y_0 .. Y_{ns-2})
-                drho = torch.empty((R, ns), dtype=torch.float64, device=dev)
-                drho[:, 0] = -rho / x[:, 0]
-                drho[:, 1:] = -(rho * mmw)[:, None] * dinvm[None, :]
-                dcpv = torch.empty((R, ns), dtype=torch.float64, device=dev)
-                dcpv[:, 0] = dcp
-                dcpv[:, 1:] = cpi[:, :-1] - cpi[:, -1:]
-                gq = (rho * cp)[:, None] * Jc[:, :, 0] + fT[:, None] * (cp[:, None] * drho + rho[:, None] * dcpv)
                 v = b[i] * lam_p
                 for j in range(i + 1, 6):
                     if A[j, i] != 0.:
                         v = v + A[j, i] * thetas[j]
-                rhs = h[:, None] * (torch.bmm(Jc, v[:, :, None])[:, :, 0] + b[i] * gq)
+                v = v.contiguous()
+                # rhs = h (J^T v + b_i grad qdot) and, for an implicit stage, I - h a_ii J^T
+                self._adjoint_prep(x, f, J, aux, cpi, dinvm, v, h, b[i], A[i, i], rhs, Mt if i > 0 else None)
                 if i > 0:
-                    Mt = (eye - (h * A[i, i])[:, None, None] * Jc.transpose(1, 2)).reshape(R, ns * ns).contiguous()
                     self.lu.factor(Mt, st)
-                    theta = self.lu.solve(rhs.contiguous(), th, st).clone()
+                    theta = self.lu.solve(rhs, th, st).clone()
                 else:
-                    theta = rhs
+                    theta = rhs.clone()
                 thetas[i] = theta
-                wv = v + A[i, i] * theta
-                wT = wv[:, 0] + b[i] * rho * cp
-                C = torch.zeros((R, ns), dtype=torch.float64, device=dev)
-                C[:, :-1] = wv[:, 1:] / rho[:, None]
-                C = C - (wT / (rho * cp))[:, None] * hs
-                grad += h[:, None] * qa * ((C * self.mw[None, :]) @ self.N)
-                L += h * b[i] * rho * cp * fT
+                self._adjoint_finish(v, theta, aux, hs, f, h, b[i], A[i, i], cm, L)
+                grad += h[:, None] * qa * (cm @ self.N)
             lam = lam_p + sum(thetas)
         out = torch.empty_like(grad)
         out[:, torch.as_tensor(self.file_order, device=dev)] = grad

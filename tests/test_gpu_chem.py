@@ -145,6 +145,42 @@ def test_batched_deterministic_and_rhs_only_kernel_agrees(mech_53_data):
     assert _scaled_err(rr.cpu().numpy().T, r1, scale) <= RTOL
 
 
+def test_every_launch_shape_gives_identical_results(mech_53_data):
+    """The launch-shape autotuner times candidate shapes on the caller's own buffers and keeps one per batch-size
+    class, which is only sound if every shape computes every state identically: each fixed shape
+    (gf_set_chem_launch_shape 0..4) gives bitwise the RHS, Jacobian and RHS-only outputs of the timed default,
+    on a batch size (3001) that leaves partial workgroups for every shape."""
+    import torch
+    from spitfire_amd import synthetic, _native
+    spec = _spec(mech_53_data)
+    ns = spec.n_species
+    x, p, _ = synthetic.random_states(ns, 3001, seed=11)
+    dev = torch.device('cuda:0')
+    xt, pt = torch.tensor(x, device=dev), torch.tensor(p, device=dev)
+    lib = _native.lib()
+
+    def run():
+        rhs, rr = torch.empty_like(xt), torch.empty_like(xt)
+        jac = torch.empty((x.shape[1], ns * ns), dtype=torch.float64, device=dev)
+        spec.griffon.batch_reactor_jac_isobaric(xt, pt, rhs, jac)
+        spec.griffon.batch_reactor_rhs_isobaric(xt, pt, rr)
+        torch.cuda.synchronize()
+        return rhs.cpu().numpy(), jac.cpu().numpy(), rr.cpu().numpy()
+
+    try:
+        _native.check(lib.gf_set_chem_launch_shape(-1))
+        ref = run()
+        for shape in range(5):
+            say('launch shape', shape)
+            _native.check(lib.gf_set_chem_launch_shape(shape))
+            got = run()
+            assert all(np.array_equal(a, b) for a, b in zip(got, ref))
+    finally:
+        _native.check(lib.gf_set_chem_launch_shape(-1))
+    with pytest.raises(_native.NativeError):
+        _native.check(lib.gf_set_chem_launch_shape(5))
+
+
 def test_full_size_batch_sampled_parity(mech_53_data, oracle_lib):
     """The 262,144-state workload: every state finite, two launches identical, a sample against the oracle."""
     import torch

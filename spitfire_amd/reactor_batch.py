@@ -201,7 +201,10 @@ class ReactorBatch:
                     self._rhs(xi, fi)
                     self._stage_residual(h, g, fi, prior, xi, xn, r, conv, True, nonlinear_solve_tolerance)
                     self.stats['newton_iterations'] += 1
-                    if bool(conv.all()):
+                    # whether all reactors have converged is read back (a host synchronization) every other
+                    # iteration: an iteration after all have converged changes nothing, every update being
+                    # masked, so the results are those of checking every iteration
+                    if (it % 2 == 1 or it == max_nonlinear_iter - 1) and bool(conv.all()):
                         break
                 else:
                     self.stats['max_newton_iterations_hit'] += 1

@@ -235,8 +235,9 @@ def main():
                     help='run one device-to-device copy of this many bytes first (counter calibration under rocprofv3)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU: this build runs the batched chemistry on a single MI355X; '
-              'independent states shard across GPUs without communication (see DESIGN.md)', file=sys.stderr)
+        print('bench.py measures one GPU only (--gpus 1): the batched chemistry, flamelets and reactors run on a '
+              'single MI355X; states, flamelets and reactors are independent and would shard across GPUs without '
+              'communication', file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

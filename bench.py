@@ -13,14 +13,20 @@ reference's own griffon build (oracle/_ref) timed on this host's cores on a boun
 baseline.
 
 Flamelets (the "flamelets" object of the JSON line): the 53-species mechanism, H2:N2 1:1 (mass) fuel against
-air, both at 300 K and 1 atm, Burke-Schumann initial states, pseudo-transient continuation to a scaled residual
-of 1e-8 with the device-resident batched solver:
-  - one flamelet, 128-point grid clustered at Z_st, chi_st = 10 1/s;
-  - 512 flamelets, grid sizes uniform in 64..256 points, chi_st log-uniform in 1e-2..1e3 1/s (seed 0);
+air, both at 300 K and 1 atm, Burke-Schumann initial states, solved to a scaled residual of 1e-8 with the
+device-resident batched solvers:
+  - one flamelet, 128-point grid clustered at Z_st, chi_st = 10 1/s, by pseudo-transient continuation;
+  - 512 flamelets, grid sizes uniform in 64..256 points, chi_st log-uniform in 1e-2..1e3 1/s (seed 0), by the
+    steady-state cascade's Newton then pseudo-transient stages (at most one restart; the time-stepping stage
+    is left out of the timed run), with converged and extinguished counts;
 each with FP64 and with FP32 block-LU factors (FP64 residuals), the agreement between the two, and the
 one-flamelet case solved by the oracle (the reference algorithm on the CPU) as the flamelet CPU baseline.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-flamelets]
+Adjoint (the "adjoint" object): 1024 reactors of the 53-species mechanism integrated to 2 ms and the discrete
+adjoint of their integrated heat release with respect to ln A of every reaction, as gradients/s; the CPU
+baseline is the cost of one such gradient by central differences of the reference step loop on the oracle.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-flamelets] [--no-adjoint] [--no-cpu-baseline]
 """
 import argparse
 import json

@@ -591,6 +591,7 @@ int gf_flamelet_batch_create(gf_mech *m, int nfl, const int *nzi, double pressur
   v.cpi = b->cpi;
   v.NV = V;
   v.NJ = Jv;
+  for (int f = 0; f < nfl; ++f) v.maxNJ = std::max(v.maxNJ, (long)ns * ((long)nzi[f] * ns + 2 * (nzi[f] - 1)));
   T(launch_fb_stencils(v, b->dz, b->ddz0, b->cmaj, b->csub, b->csup, b->mco, b->nco, st));
   if (rc == GF_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(GF_ERR_HIP, "stream synchronize failed");
   if (rc != GF_OK) {

@@ -279,14 +279,15 @@ __global__ void fb_rhs_kernel(FlameletBatchView b, const double *X, const int *f
 //   mode 2  prefactor * J - I  implicit time stepping (flamelet.py:756-762, flamelet_kernels.cpp:1395-1408), with
 //                              the prefactor of flamelet f in vals[f] (or the scalar prefactor when vals is null)
 // vals: mode 0 the local pseudo-time steps (dof layout), mode 2 per-flamelet prefactors
+// Grid: y = flamelet, x over that flamelet's entries (blocks past its end, or of an unflagged flamelet, exit).
 __global__ void fb_matrix_kernel(MechView mv, FlameletBatchView b, const double *X, const int *flag, int mode,
                                  const double *vals, double prefactor, double *J) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= b.NJ) return;
-  const int f = fb_find(b.voff, b.B, e);
+  const int f = blockIdx.y;
   if (!flag[f]) return;
+  const long el = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= (f + 1 < b.B ? b.voff[f + 1] : b.NJ) - b.voff[f]) return;
+  const long e = b.voff[f] + el;
   const FlameletIO io = fb_io(b, f, X);
-  const long el = e - b.voff[f];
   double v = el < (long)io.nzi * io.ns * io.ns ? J[e] : 0.;
   int dr;
   jac_entry(mv, io, el, mode == 2, mode == 2 && vals ? vals[f] : prefactor, 0, v, dr);
@@ -468,8 +469,12 @@ int launch_fb_rhs(const FlameletBatchView &b, const double *X, const int *flag, 
   FB_LAUNCH(fb_rhs_kernel, b.NV, b, X, flag, R);
 }
 int launch_fb_matrix(const gf_mech *m, const FlameletBatchView &b, const double *X, const int *flag, int mode,
-                     const double *ds, double prefactor, double *J, hipStream_t st) {
-  FB_LAUNCH(fb_matrix_kernel, b.NJ, m->view, b, X, flag, mode, ds, prefactor, J);
+                     const double *vals, double prefactor, double *J, hipStream_t st) {
+  if (b.maxNJ > 0)
+    hipLaunchKernelGGL(fb_matrix_kernel, dim3(nblk(b.maxNJ, 256), (unsigned)b.B), dim3(256), 0, st, m->view, b, X, flag,
+                       mode, vals, prefactor, J);
+  if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, "fb_matrix_kernel launch failed");
+  return GF_OK;
 }
 int launch_fb_unpack(const FlameletBatchView &b, const int *flag, const long *pk, const double *packed, double *dst,
                      int interior, hipStream_t st) {

@@ -191,6 +191,7 @@ struct FlameletBatchView {
   const double *cmaj = nullptr, *csub = nullptr, *csup = nullptr, *mco = nullptr, *nco = nullptr, *chi = nullptr;
   const double *chem_rhs = nullptr, *aux = nullptr, *cpi = nullptr;
   long NV = 0, NJ = 0;  // total unknowns, total Jacobian entries
+  long maxNJ = 0;       // Jacobian entries of the largest flamelet
 };
 int launch_fb_stencils(const FlameletBatchView &b, const double *dz, const long *dz0, double *cmaj, double *csub,
                        double *csup, double *mco, double *nco, hipStream_t st);

@@ -60,6 +60,8 @@ class SimpleNewtonSolver:
                 setup_method(solution)
                 setups += 1
             dstate = solve_method(residual)
+            if isinstance(dstate, tuple):  # the reference's solve methods return (update, iterations, converged)
+                dstate = dstate[0]
             self._check(dstate, 'solution update check!')
             solution -= dstate
             residual, rhs = residual_method(solution, evaluate_new_rhs=True)
@@ -77,6 +79,46 @@ class StepOutput:
     def __init__(self, **kw):
         for k, v in kw.items():
             setattr(self, k, v)
+
+
+class BackwardEulerS1P1Q1:
+    """Backward Euler, x_{n+1} = x_n + dt f(t + dt, x_{n+1}), with the error estimate of methods.py:339-398: the
+    difference between a one-iteration Newton predictor and the converged solution. As in the reference, the
+    second solve starts from x_n but takes the predictor's right-hand side as its initial one."""
+    name = 'backward Euler'
+    order = 1
+    is_implicit = True
+    is_adaptive = True
+    implicit_coefficient = 1.
+
+    def __init__(self, nonlinear_solver, norm_weighting=1., norm_order=_INF):
+        self.nonlinear_solver = nonlinear_solver
+        self.norm_weighting = norm_weighting
+        self.norm_order = norm_order
+
+    def norm(self, x):
+        return np.linalg.norm(x * self.norm_weighting, ord=self.norm_order)
+
+    def single_step(self, state, t, dt, rhs, lhs_setup, lhs_solve):
+        x_n = np.copy(state)
+
+        def residual(xa, existing_rhs=None, evaluate_new_rhs=True):
+            f = rhs(t + dt, xa) if evaluate_new_rhs else existing_rhs
+            return dt * f - (xa - x_n), f
+
+        ns = self.nonlinear_solver
+        cap = ns.max_nonlinear_iter
+        ns.max_nonlinear_iter = 1
+        try:
+            pred = ns(residual_method=residual, setup_method=lambda xa: lhs_setup(t + dt, xa), solve_method=lhs_solve,
+                      initial_guess=state, initial_rhs=rhs(t + dt, state))
+        finally:
+            ns.max_nonlinear_iter = cap
+        out = ns(residual_method=residual, setup_method=lambda xa: lhs_setup(t + dt, xa), solve_method=lhs_solve,
+                 initial_guess=state, initial_rhs=pred.rhs_at_converged)
+        return StepOutput(solution_update=out.solution - state, temporal_error=self.norm(out.solution - pred.solution),
+                          nonlinear_iter=out.iter, nonlinear_converged=out.converged,
+                          slow_nonlinear_convergence=out.slow_convergence, projector_setups=out.projector_setups)
 
 
 class KennedyCarpenterS6P4Q3:
@@ -139,7 +181,13 @@ class KennedyCarpenterS6P4Q3:
 
 
 class PIController:
-    """PI control of the step size on the embedded error estimate (stepcontrol.py:50-117)."""
+    """PI control of the step size on the embedded error estimate e (stepcontrol.py:50-117):
+
+        dt_new = min(dt * min(max_ramp, (target / e)^ki * (e_rec / e)^kp), max_step)
+
+    where e_rec is the most recently recorded error. The reference records the current error before it forms
+    the proportional factor, so e_rec = e and that factor is exactly 1; this controller records the same way.
+    An error below 1e-16 ramps the step by max_ramp without recording it."""
 
     def __init__(self, kp=0.06666666667, ki=0.1333333333, target_error=1.e-4, max_step=1.e4, max_ramp=1.1,
                  first_step=1.e-3):
@@ -148,33 +196,30 @@ class PIController:
         self._max_step = max_step
         self._max_ramp = max_ramp
         self._first_step = first_step
-        self._err_history = np.zeros(2)
-        self._step_history = np.zeros(2)
+        self._recorded_error = None
+        self._recorded_step = None
 
-    def __call__(self, step_count, step, step_output):
-        error = step_output.temporal_error
-        if error < 1.e-16:
+    def __call__(self, step_count, step, step_output, *args, **kwargs):
+        e = step_output.temporal_error
+        if e < 1.e-16:
             return min(step * self._max_ramp, self._max_step)
-        if step_count < 1:
-            self._err_history[step_count] = error
-            self._step_history[step_count] = step
-        else:
-            self._err_history[:-1] = self._err_history[1:]
-            self._step_history[:-1] = self._step_history[1:]
-            self._err_history[-1] = error
-            self._step_history[-1] = step
-        if step_count == 0:
-            mod = min(self._max_ramp, (self._target_error / error) ** self._ki)
-        else:
-            mod = min(self._max_ramp,
-                      (self._target_error / error) ** self._ki * (self._err_history[-1] / error) ** self._kp)
-        return min(step * mod, self._max_step)
+        self._recorded_error, self._recorded_step = e, step
+        factor = (self._target_error / e) ** self._ki
+        if step_count > 0:
+            factor = factor * (self._recorded_error / e) ** self._kp
+        return min(step * min(self._max_ramp, factor), self._max_step)
 
     def first_step_size(self):
         return self._first_step
 
+    def last_step_size(self):
+        return self._recorded_step
+
     def target_error(self):
         return self._target_error
+
+    def step_size_is_constant(self):
+        return False
 
 
 def odesolve(right_hand_side, initial_state, initial_time=0., stop_at_time=None, stop_at_steady=None,

@@ -71,6 +71,29 @@ int gf_reactor_jac_isobaric(gf_mech *m, const double *state, double pressure, do
                             double surface_area_over_volume, int heat_transfer_option, int open,
                             int rates_sensitivity_option, int sensitivity_transform_option, double *out_rhs,
                             double *out_jac);                                                       /* griffon.pyx:807-824 */
+/* Isochoric reactor, state [rho, T, Y_0 .. Y_{ns-2}]; rhs has ns+1 entries, jac (ns+1)^2 column-major. */
+int gf_reactor_rhs_isochoric(gf_mech *m, const double *state, double inflow_density, double inflow_T,
+                             const double *inflow_y, double tau, double fluid_T, double surf_T, double h_conv,
+                             double eps_rad, double surface_area_over_volume, int heat_transfer_option, int open,
+                             double *out_rhs);                                                      /* griffon.pyx:831-845 */
+int gf_reactor_jac_isochoric(gf_mech *m, const double *state, double inflow_density, double inflow_T,
+                             const double *inflow_y, double tau, double fluid_T, double surf_T, double h_conv,
+                             double eps_rad, double surface_area_over_volume, int heat_transfer_option, int open,
+                             int rates_sensitivity_option, double *out_rhs, double *out_jac);        /* griffon.pyx:850-866 */
+/* Thermodynamic getters. Scalars come back through out_value. y holds all ns mass fractions. */
+int gf_mixture_molecular_weight(gf_mech *m, const double *y, double *out_value);                    /* griffon.pyx:684-688 */
+int gf_mole_fractions(gf_mech *m, const double *y, double *out_x);                                  /* griffon.pyx:690-694 */
+int gf_ideal_gas_density(gf_mech *m, double p, double T, const double *y, double *out_value);       /* griffon.pyx:696-700 */
+int gf_ideal_gas_pressure(gf_mech *m, double rho, double T, const double *y, double *out_value);    /* griffon.pyx:702-706 */
+int gf_cp_mix(gf_mech *m, double T, const double *y, double *out_value);                            /* griffon.pyx:708-712 */
+int gf_cv_mix(gf_mech *m, double T, const double *y, double *out_value);                            /* griffon.pyx:714-718 */
+int gf_species_cp(gf_mech *m, double T, double *out_cp);                                            /* griffon.pyx:720-724 */
+int gf_species_cv(gf_mech *m, double T, double *out_cv);                                            /* griffon.pyx:726-730 */
+int gf_enthalpy_mix(gf_mech *m, double T, const double *y, double *out_value);                      /* griffon.pyx:732-736 */
+int gf_energy_mix(gf_mech *m, double T, const double *y, double *out_value);                        /* griffon.pyx:738-742 */
+int gf_species_enthalpies(gf_mech *m, double T, double *out_h);                                     /* griffon.pyx:744-748 */
+int gf_species_energies(gf_mech *m, double T, double *out_e);                                       /* griffon.pyx:750-754 */
+int gf_dcpdT_species(gf_mech *m, double T, const double *y, double *out_dcpdT);                     /* griffon.pyx:756-760 */
 int gf_flamelet_stencils(gf_mech *m, const double *dz, int nzi, const double *dissipation_rate,
                          const double *inv_lewis, double *out_cmajor, double *out_csub, double *out_csup,
                          double *out_mcoeff, double *out_ncoeff);                                   /* griffon.pyx:556-566 */

@@ -400,7 +400,11 @@ double mass_action_deriv(const std::vector<int> &idx, const std::vector<int> &st
 
 // rates_sensitivities_exact.cpp:33-1028 — option-0 exact sensitivities, wsens[(ns+1)^2] column-major
 // (col 0 d/drho, col 1 d/dT, col 2+s d/dY_s; row = species; stride ns+1)
-void prod_rates_sens_exact(const Mech &m, double T, double rho, double Mmix, const double *y, double *w, double *ws) {
+// notbaf: rates_sensitivities_no_tbaf_sens.cpp (option 1), the same rates with the derivatives of the third-body
+// and falloff multiplier left out: dq = dRnet * Ctbaf (no_tbaf_sens.cpp:788-795). Special orders are not restated for
+// option 1 (that file has no special-order branch).
+void prod_rates_sens_exact(const Mech &m, double T, double rho, double Mmix, const double *y, double *w, double *ws,
+                           bool notbaf = false) {
   const int ns = m.ns;
   const int nsp1 = ns + 1;
   for (int i = 0; i < ns; ++i) w[i] = 0.;
@@ -562,6 +566,12 @@ void prod_rates_sens_exact(const Mech &m, double T, double rho, double Mmix, con
         if (r.tb_idx[i] == ns - 1) { for (int s = 0; s < ns - 1; ++s) dCdY[s] -= nsTmp * r.tb_eff[i]; break; }
     }
 
+    if (notbaf) {
+      if (r.hasOrders) throw std::runtime_error("rates sensitivity option 1 with special orders is not supported");
+      dCdrho = 0.;
+      dCdT = 0.;
+      for (int s = 0; s < ns - 1; ++s) dCdY[s] = 0.;
+    }
     const double q = Rnet * Ctbaf;
     const double dqdrho = dRdrho * Ctbaf + dCdrho * Rnet;
     const double dqdT = dRdT * Ctbaf + dCdT * Rnet;
@@ -681,7 +691,7 @@ void reactor_rhs_isobaric(const Mech &m, const double *state, double p, double T
 // isobaric_reactor_kernels.cpp:221-317 (rates_sensitivity_option 0, transform option 0)
 void reactor_jac_isobaric(const Mech &m, const double *state, double p, double Tin, const double *yin, double tau,
                           double Tf, double Ts, double hc, double eps, double sov, int heat, bool open,
-                          double *rhs, double *jac) {
+                          double *rhs, double *jac, int rates_opt = 0) {
   const int n = m.ns;
   std::vector<double> y(n), h(n), w(n), cpi(n), cpis(n), ws((n + 1) * (n + 1)), pj(n * (n + 1) + 1);
   const double T = state[0];
@@ -692,7 +702,7 @@ void reactor_jac_isobaric(const Mech &m, const double *state, double p, double T
   cp_mix_and_species(m, T, y.data(), &cp, cpi.data());
   species_enthalpies(m, T, h.data());
   cp_sens_T(m, T, y.data(), &cps, cpis.data());
-  prod_rates_sens_exact(m, T, rho, mmw, y.data(), w.data(), ws.data());
+  prod_rates_sens_exact(m, T, rho, mmw, y.data(), w.data(), ws.data(), rates_opt == 1);
   chem_jac_isobaric(m, rho, cp, cpi.data(), cps, h.data(), w.data(), ws.data(), rhs, pj.data());
   if (open) {
     std::vector<double> mr(n), mj(n * (n + 1) + 1), hin(n);
@@ -759,6 +769,151 @@ double cp_mix_only(const Mech &m, double T, const double *y) {
   double cp;
   cp_mix_and_species(m, T, y, &cp, cpi.data());
   return cp;
+}
+
+// thermodynamics_kernels.cpp:353-365 — species internal energies e_i = h_i - R T / M_i
+void species_energies(const Mech &m, double T, double *e) {
+  species_enthalpies(m, T, e);
+  const double RT = m.Ru * T;
+  for (int i = 0; i < m.ns; ++i) e[i] -= RT * m.invmw[i];
+}
+
+// thermodynamics_kernels.cpp:169-181 — cv of the mixture and of the species: cp - R / M
+void cv_mix_and_species(const Mech &m, double T, const double *y, double mmw, double *cv, double *cvi) {
+  cp_mix_and_species(m, T, y, cv, cvi);
+  *cv -= m.Ru / mmw;
+  for (int i = 0; i < m.ns; ++i) cvi[i] -= m.Ru * m.invmw[i];
+}
+
+// isochoric_reactor_kernels.cpp:18-29
+void chem_rhs_isochoric(const Mech &m, double rho, double cv, const double *e, const double *w, double *rhs) {
+  const int n = m.ns;
+  rhs[0] = 0.;
+  rhs[1] = -dotp(n, e, w) / (rho * cv);
+  const double ir = 1. / rho;
+  for (int i = 0; i < n - 1; ++i) rhs[2 + i] = w[i] * ir;
+}
+
+// isochoric_reactor_kernels.cpp:31-37
+inline double heat_rhs_isochoric(double T, double rho, double cv, double Tf, double Ts, double hc, double eps,
+                                 double sov) {
+  return sov / (rho * cv) * (hc * (Tf - T) + eps * 5.67e-8 * (Ts * Ts * Ts * Ts - T * T * T * T));
+}
+
+// isochoric_reactor_kernels.cpp:39-58 (the inflow and mixture pressures are not used by the terms)
+void mass_rhs_isochoric(const Mech &m, const double *y, const double *e, const double *ein, double rho, double rhoin,
+                        double cv, const double *yin, double tau, double *rhs) {
+  const int n = m.ns;
+  rhs[1] = (ein[n - 1] - e[n - 1]) * yin[n - 1];
+  for (int i = 0; i < n - 1; ++i) { rhs[1] += (ein[i] - e[i]) * yin[i]; rhs[2 + i] = yin[i] - y[i]; }
+  const double ir = 1. / rho, it = 1. / tau;
+  rhs[0] = (rhoin - rho) * it;
+  rhs[1] /= cv;
+  for (int i = 1; i < n + 1; ++i) rhs[i] *= it * rhoin * ir;
+}
+
+// isochoric_reactor_kernels.cpp:60-104 — (ns+1) x (ns+1) column-major Jacobian in (rho, T, Y)
+void chem_jac_isochoric(const Mech &m, double rho, double cv, const double *cvi, double cvsT, const double *e,
+                        const double *w, const double *ws, double *rhs, double *J) {
+  const int n = m.ns, np = n + 1;
+  chem_rhs_isochoric(m, rho, cv, e, w, rhs);
+  const double ir = 1. / rho, icv = 1. / cv, irc = 1. / (rho * cv);
+  J[0] = 0.;
+  J[1] = -irc * dotp(n, e, ws) - ir * rhs[1];
+  for (int i = 0; i < n - 1; ++i) J[2 + i] = ir * (ws[i] - ir * w[i]);
+  J[np] = 0.;
+  J[np + 1] = -icv * (ir * (dotp(n, e, &ws[np]) + dotp(n, cvi, w)) + rhs[1] * cvsT);
+  for (int i = 0; i < n - 1; ++i) J[np + 2 + i] = ir * ws[np + i];
+  const double cvn = cvi[n - 1];
+  for (int k = 0; k < n - 1; ++k) {
+    const int c = (2 + k) * np;
+    J[c] = 0.;
+    J[c + 1] = -irc * dotp(n, e, &ws[(2 + k) * np]) - rhs[1] * (cvi[k] - cvn) * icv;
+    for (int i = 0; i < n - 1; ++i) J[c + 2 + i] = ir * ws[(2 + k) * np + i];
+  }
+}
+
+// isochoric_reactor_kernels.cpp:106-154
+void mass_jac_isochoric(const Mech &m, const double *y, double rho, double rhoin, double cv, double cvsT,
+                        const double *cvi, const double *e, const double *ein, const double *yin, double tau,
+                        double *rhs, double *J) {
+  const int n = m.ns, np = n + 1;
+  mass_rhs_isochoric(m, y, e, ein, rho, rhoin, cv, yin, tau, rhs);
+  const double ir = 1. / rho, icv = 1. / cv, it = 1. / tau;
+  J[0] = -it;
+  for (int i = 0; i < n; ++i) J[1 + i] = -ir * rhs[1 + i];
+  J[np] = 0.;
+  J[np + 1] = -icv * (it * ir * (rhoin * dotp(n, cvi, yin)) + cvsT * rhs[1]);
+  for (int i = 0; i < n - 1; ++i) J[np + 2 + i] = 0.;
+  const double cvn = cvi[n - 1];
+  for (int k = 0; k < n - 1; ++k) {
+    const int c = (2 + k) * np;
+    J[c] = 0.;
+    J[c + 1] = -icv * rhs[1] * (cvi[k] - cvn);
+    for (int i = 0; i < n - 1; ++i) J[c + 2 + i] = 0.;
+    J[c + 2 + k] = -it * rhoin / rho;
+  }
+}
+
+// isochoric_reactor_kernels.cpp:186-239 — state [rho, T, Y_0 .. Y_{ns-2}]
+void reactor_rhs_isochoric(const Mech &m, const double *state, double rhoin, double Tin, const double *yin, double tau,
+                           double Tf, double Ts, double hc, double eps, double sov, int heat, bool open, double *rhs) {
+  const int n = m.ns;
+  std::vector<double> y(n), e(n), w(n), cvi(n);
+  const double rho = state[0], T = state[1];
+  extract_y(m, &state[2], y.data());
+  const double mmw = mmw_of(m, y.data());
+  double cv;
+  cv_mix_and_species(m, T, y.data(), mmw, &cv, cvi.data());
+  species_energies(m, T, e.data());
+  production_rates(m, T, rho, mmw, y.data(), w.data());
+  chem_rhs_isochoric(m, rho, cv, e.data(), w.data(), rhs);
+  if (open) {
+    std::vector<double> ein(n), mr(n + 1);
+    species_energies(m, Tin, ein.data());
+    mass_rhs_isochoric(m, y.data(), e.data(), ein.data(), rho, rhoin, cv, yin, tau, mr.data());
+    for (int i = 0; i < n + 1; ++i) rhs[i] += mr[i];
+  }
+  if (heat == 1) rhs[1] = 0.;
+  else if (heat == 2) rhs[1] += heat_rhs_isochoric(T, rho, cv, Tf, Ts, hc, eps, sov);
+}
+
+// isochoric_reactor_kernels.cpp:241-333
+void reactor_jac_isochoric(const Mech &m, const double *state, double rhoin, double Tin, const double *yin, double tau,
+                           double Tf, double Ts, double hc, double eps, double sov, int heat, bool open, int rates_opt,
+                           double *rhs, double *J) {
+  const int n = m.ns, np = n + 1;
+  std::vector<double> y(n), e(n), w(n), cvi(n), cvis(n), ws(np * np);
+  const double rho = state[0], T = state[1];
+  extract_y(m, &state[2], y.data());
+  const double mmw = mmw_of(m, y.data());
+  double cv, cvsT;
+  cv_mix_and_species(m, T, y.data(), mmw, &cv, cvi.data());
+  cp_sens_T(m, T, y.data(), &cvsT, cvis.data());
+  species_energies(m, T, e.data());
+  prod_rates_sens_exact(m, T, rho, mmw, y.data(), w.data(), ws.data(), rates_opt == 1);
+  chem_jac_isochoric(m, rho, cv, cvi.data(), cvsT, e.data(), w.data(), ws.data(), rhs, J);
+  if (open) {
+    std::vector<double> ein(n), mr(np), mj(np * np);
+    species_energies(m, Tin, ein.data());
+    mass_jac_isochoric(m, y.data(), rho, rhoin, cv, cvsT, cvi.data(), e.data(), ein.data(), yin, tau, mr.data(),
+                       mj.data());
+    for (int i = 0; i < np; ++i) rhs[i] += mr[i];
+    for (int i = 0; i < np * np; ++i) J[i] += mj[i];
+  }
+  if (heat == 1) {
+    for (int k = 0; k < np; ++k) J[k * np + 1] = 0.;
+    rhs[1] = 0.;
+  } else if (heat == 2) {
+    // heat_jac_isochoric, isochoric_reactor_kernels.cpp:156-184
+    const double q = heat_rhs_isochoric(T, rho, cv, Tf, Ts, hc, eps, sov);
+    const double irc = 1. / (rho * cv), icv = 1. / cv;
+    rhs[1] += q;
+    J[1] += -q / rho;
+    J[np + 1] += -icv * cvsT * q - sov * irc * (hc + 4. * eps * 5.67e-8 * T * T * T);
+    const double cvn = cvi[n - 1];
+    for (int k = 0; k < n - 1; ++k) J[(2 + k) * np + 1] += icv * q * (cvn - cvi[k]);
+  }
 }
 
 // --- dense LU with partial pivoting (LAPACK dgetrf/dgetrs semantics: 1-based pivots, first max |a|) ---
@@ -943,6 +1098,71 @@ int gfo_reactor_jac_isobaric(void *h, const double *state, double p, double Tin,
                              double *rhs, double *jac) {
   try { reactor_jac_isobaric(*static_cast<Mech *>(h), state, p, Tin, yin, tau, Tf, Ts, hc, eps, sov, heat, open != 0, rhs, jac); }
   catch (...) { return 2; }
+  return 0;
+}
+
+// prod_rates_primitive_sensitivities with rates_sensitivity_option 0 (exact), 1 (no third-body/falloff
+// derivatives) or 2 (the sparse formulation of the same exact values)
+int gfo_prod_rates_sens_opt(void *h, double rho, double T, const double *y, int option, double *ws) {
+  const Mech &m = *static_cast<Mech *>(h);
+  std::vector<double> w(m.ns);
+  try { prod_rates_sens_exact(m, T, rho, mmw_of(m, y), y, w.data(), ws, option == 1); } catch (...) { return 2; }
+  return 0;
+}
+
+int gfo_reactor_jac_isobaric_opt(void *h, const double *state, double p, double Tin, const double *yin, double tau,
+                                 double Tf, double Ts, double hc, double eps, double sov, int heat, int open,
+                                 int rates_opt, double *rhs, double *jac) {
+  try {
+    reactor_jac_isobaric(*static_cast<Mech *>(h), state, p, Tin, yin, tau, Tf, Ts, hc, eps, sov, heat, open != 0, rhs,
+                         jac, rates_opt);
+  } catch (...) { return 2; }
+  return 0;
+}
+
+int gfo_reactor_rhs_isochoric(void *h, const double *state, double rhoin, double Tin, const double *yin, double tau,
+                              double Tf, double Ts, double hc, double eps, double sov, int heat, int open, double *rhs) {
+  try {
+    reactor_rhs_isochoric(*static_cast<Mech *>(h), state, rhoin, Tin, yin, tau, Tf, Ts, hc, eps, sov, heat, open != 0,
+                          rhs);
+  } catch (...) { return 2; }
+  return 0;
+}
+
+int gfo_reactor_jac_isochoric(void *h, const double *state, double rhoin, double Tin, const double *yin, double tau,
+                              double Tf, double Ts, double hc, double eps, double sov, int heat, int open,
+                              int rates_opt, double *rhs, double *jac) {
+  try {
+    reactor_jac_isochoric(*static_cast<Mech *>(h), state, rhoin, Tin, yin, tau, Tf, Ts, hc, eps, sov, heat, open != 0,
+                          rates_opt, rhs, jac);
+  } catch (...) { return 2; }
+  return 0;
+}
+
+// The thermodynamic getters of griffon.pyx:684-758 at (T, y) and, for the equation of state, pressure p and density
+// rho: sc = {mixture molecular weight, ideal-gas density(p, T, y), ideal-gas pressure(rho, T, y), cp_mix, cv_mix,
+// enthalpy_mix, energy_mix, d(cp_mix)/dT}; x mole fractions; species cp, cv, h, e and d(cp)/dT
+// (thermodynamics_kernels.cpp:18-44, 131-181, 183-260, 353-383; combustion_kernels.h:381-387, 527-534).
+int gfo_thermo_getters(void *h, double T, const double *y, double p, double rho, double *sc, double *x, double *cpi,
+                       double *cvi, double *hi, double *ei, double *cpis) {
+  const Mech &m = *static_cast<Mech *>(h);
+  const int n = m.ns;
+  const double mmw = mmw_of(m, y);
+  for (int i = 0; i < n; ++i) x[i] = y[i] * mmw * m.invmw[i];
+  double cp, cv, cps;
+  cp_mix_and_species(m, T, y, &cp, cpi);
+  cv_mix_and_species(m, T, y, mmw, &cv, cvi);
+  cp_sens_T(m, T, y, &cps, cpis);
+  species_enthalpies(m, T, hi);
+  species_energies(m, T, ei);
+  sc[0] = mmw;
+  sc[1] = p * mmw / (T * m.Ru);
+  sc[2] = rho * T * m.Ru / mmw;
+  sc[3] = cp;
+  sc[4] = cv;
+  sc[5] = dotp(n, hi, y);
+  sc[6] = dotp(n, ei, y);
+  sc[7] = cps;
   return 0;
 }
 

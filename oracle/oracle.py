@@ -154,6 +154,51 @@ class CpuGriffon:
                                                _dp(rhs), _dp(jac)) == 0
         return rhs, jac
 
+    def prod_rates_sens_opt(self, rho, T, y, option):
+        ws = np.zeros((self.ns + 1) ** 2)
+        rc = self._f('prod_rates_sens_opt')(self.h, ctypes.c_double(rho), ctypes.c_double(T),
+                                            _dp(np.ascontiguousarray(y, dtype=np.float64)), int(option), _dp(ws))
+        if rc != 0:
+            raise ValueError('rates sensitivity option %d rejected (code %d)' % (option, rc))
+        return ws
+
+    def reactor_jac_isobaric_opt(self, state, p, Tin=0., yin=None, tau=0., Tf=0., Ts=0., hc=0., eps=0., sov=0.,
+                                 heat=0, open_=False, rates_opt=0):
+        rhs = np.zeros(self.ns)
+        jac = np.zeros(self.ns * self.ns)
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        assert self._f('reactor_jac_isobaric_opt')(self.h, _dp(st), *self._reactor_args(p, Tin, yin, tau, Tf, Ts, hc,
+                                                                                          eps, sov, heat, open_),
+                                                   int(rates_opt), _dp(rhs), _dp(jac)) == 0
+        return rhs, jac
+
+    def reactor_rhs_isochoric(self, state, rho_in=0., Tin=0., yin=None, tau=0., Tf=0., Ts=0., hc=0., eps=0., sov=0.,
+                              heat=0, open_=False):
+        rhs = np.zeros(self.ns + 1)
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        assert self._f('reactor_rhs_isochoric')(self.h, _dp(st), *self._reactor_args(rho_in, Tin, yin, tau, Tf, Ts, hc,
+                                                                                       eps, sov, heat, open_),
+                                                _dp(rhs)) == 0
+        return rhs
+
+    def reactor_jac_isochoric(self, state, rho_in=0., Tin=0., yin=None, tau=0., Tf=0., Ts=0., hc=0., eps=0., sov=0.,
+                              heat=0, open_=False, rates_opt=0):
+        rhs = np.zeros(self.ns + 1)
+        jac = np.zeros((self.ns + 1) ** 2)
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        assert self._f('reactor_jac_isochoric')(self.h, _dp(st), *self._reactor_args(rho_in, Tin, yin, tau, Tf, Ts, hc,
+                                                                                       eps, sov, heat, open_),
+                                                int(rates_opt), _dp(rhs), _dp(jac)) == 0
+        return rhs, jac
+
+    def thermo_getters(self, T, y, p, rho):
+        """{mmw, density, pressure, cp, cv, h, e, dcpdT} and mole fractions, species cp, cv, h, e, dcp/dT"""
+        sc = np.zeros(8)
+        arrs = [np.zeros(self.ns) for _ in range(6)]
+        assert self._f('thermo_getters')(self.h, ctypes.c_double(T), _dp(np.ascontiguousarray(y, dtype=np.float64)),
+                                         ctypes.c_double(p), ctypes.c_double(rho), _dp(sc), *[_dp(a) for a in arrs]) == 0
+        return sc, arrs
+
     # --- batched (closed adiabatic isobaric), state SoA [ns, N] ---
     def batch_rhs_jac(self, stateSoA, p, nthreads=0):
         N = stateSoA.shape[1]

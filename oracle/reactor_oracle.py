@@ -1,6 +1,7 @@
 """TEST INFRASTRUCTURE ONLY — a HomogeneousReactor whose RHS and Jacobian run on the CPU oracle
 (oracle/griffon_oracle.cpp) instead of the GPU: the checker for reactor integrations. The integrator
-(ESDIRK64 + Newton + PI control) is the product's spitfire_amd.reactors.HomogeneousReactor, unchanged."""
+(ESDIRK64 + Newton + PI control) is the product's spitfire_amd.reactors.HomogeneousReactor, unchanged; it is
+itself pinned to the reference's gold trajectories (tests/test_reference_gold.py)."""
 from oracle.oracle import CpuGriffon
 from spitfire_amd.mechdata import CompiledMechanism
 from spitfire_amd.reactors import HomogeneousReactor
@@ -11,10 +12,21 @@ class OracleReactor(HomogeneousReactor):
         super().__init__(mech_spec, initial_mixture, **kwargs)
         self._cpu = cpu if cpu is not None else CpuGriffon(CompiledMechanism(mech_spec.mech_data, order='name'))
 
-    def _rhs(self, t, state):
-        return self._cpu.reactor_rhs_isobaric(state, self._initial_pressure, heat=self._heat_transfer_option)
+    def _cpu_args(self, t):
+        first, Tin, yin, tau, Tc, Tr, hc, eps, sov, heat, open_ = self._griffon_args(t)
+        return first, dict(Tin=Tin, yin=yin, tau=tau, Tf=Tc, Ts=Tr, hc=hc, eps=eps, sov=sov, heat=heat, open_=open_)
 
-    def _jac(self, state):
+    def _rhs(self, t, state):
+        first, kw = self._cpu_args(t)
+        if self._configuration == 'isobaric':
+            return self._cpu.reactor_rhs_isobaric(state, first, **kw)
+        return self._cpu.reactor_rhs_isochoric(state, rho_in=first, **kw)
+
+    def _jac(self, state, t=None):
         n = self._n_equations
-        return self._cpu.reactor_jac_isobaric(state, self._initial_pressure,
-                                              heat=self._heat_transfer_option)[1].reshape((n, n), order='F')
+        first, kw = self._cpu_args(self._current_time if t is None else t)
+        if self._configuration == 'isobaric':
+            j = self._cpu.reactor_jac_isobaric_opt(state, first, rates_opt=self._rates_sensitivity_option, **kw)[1]
+        else:
+            j = self._cpu.reactor_jac_isochoric(state, rho_in=first, rates_opt=self._rates_sensitivity_option, **kw)[1]
+        return j.reshape((n, n), order='F')

@@ -135,6 +135,56 @@ int gfr_reactor_jac_isobaric(void *h, const double *state, double p, double Tin,
   return 0;
 }
 
+int gfr_prod_rates_sens_opt(void *h, double rho, double T, const double *y, int option, double *ws) {
+  try { static_cast<CombustionKernels *>(h)->prod_rates_primitive_sensitivities(rho, T, y, option, ws); }
+  catch (...) { return 2; }
+  return 0;
+}
+
+int gfr_reactor_jac_isobaric_opt(void *h, const double *state, double p, double Tin, const double *yin, double tau,
+                                 double Tf, double Ts, double hc, double eps, double sov, int heat, int open,
+                                 int rates_opt, double *rhs, double *jac) {
+  static_cast<CombustionKernels *>(h)->reactor_jac_isobaric(state, p, Tin, yin, tau, Tf, Ts, hc, eps, sov, heat,
+                                                            open != 0, rates_opt, 0, rhs, jac);
+  return 0;
+}
+
+int gfr_reactor_rhs_isochoric(void *h, const double *state, double rhoin, double Tin, const double *yin, double tau,
+                              double Tf, double Ts, double hc, double eps, double sov, int heat, int open, double *rhs) {
+  static_cast<CombustionKernels *>(h)->reactor_rhs_isochoric(state, rhoin, Tin, yin, tau, Tf, Ts, hc, eps, sov, heat,
+                                                             open != 0, rhs);
+  return 0;
+}
+
+int gfr_reactor_jac_isochoric(void *h, const double *state, double rhoin, double Tin, const double *yin, double tau,
+                              double Tf, double Ts, double hc, double eps, double sov, int heat, int open,
+                              int rates_opt, double *rhs, double *jac) {
+  static_cast<CombustionKernels *>(h)->reactor_jac_isochoric(state, rhoin, Tin, yin, tau, Tf, Ts, hc, eps, sov, heat,
+                                                             open != 0, rates_opt, rhs, jac);
+  return 0;
+}
+
+int gfr_thermo_getters(void *h, double T, const double *y, double p, double rho, double *sc, double *x, double *cpi,
+                       double *cvi, double *hi, double *ei, double *cpis) {
+  auto *k = static_cast<CombustionKernels *>(h);
+  double cps;
+  sc[0] = k->mixture_molecular_weight(y);
+  k->mole_fractions(y, x);
+  sc[1] = k->ideal_gas_density(p, T, y);
+  sc[2] = k->ideal_gas_pressure(rho, T, y);
+  sc[3] = k->cp_mix(T, y);
+  sc[4] = k->cv_mix(T, y);
+  sc[5] = k->enthalpy_mix(T, y);
+  sc[6] = k->energy_mix(T, y);
+  k->species_cp(T, cpi);
+  k->species_cv(T, cvi);
+  k->species_enthalpies(T, hi);
+  k->species_energies(T, ei);
+  k->cp_sens_T(T, y, &cps, cpis);
+  sc[7] = cps;
+  return 0;
+}
+
 // OpenMP over independent states (griffon's compute methods are const and use only stack storage).
 int gfr_batch_rhs_jac(void *h, int N, const double *stateSoA, const double *p, double *rhs, double *jac, int nthreads) {
   auto *k = static_cast<CombustionKernels *>(h);

@@ -37,6 +37,22 @@ SIGNATURES = {
     'gf_reactor_rhs_isobaric': [_V, _D, _dbl, _dbl, _D, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _int, _D],
     'gf_reactor_jac_isobaric': [_V, _D, _dbl, _dbl, _D, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _int, _int, _int,
                                 _D, _D],
+    'gf_reactor_rhs_isochoric': [_V, _D, _dbl, _dbl, _D, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _int, _D],
+    'gf_reactor_jac_isochoric': [_V, _D, _dbl, _dbl, _D, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _int, _int, _D,
+                                 _D],
+    'gf_mixture_molecular_weight': [_V, _D, _D],
+    'gf_mole_fractions': [_V, _D, _D],
+    'gf_ideal_gas_density': [_V, _dbl, _dbl, _D, _D],
+    'gf_ideal_gas_pressure': [_V, _dbl, _dbl, _D, _D],
+    'gf_cp_mix': [_V, _dbl, _D, _D],
+    'gf_cv_mix': [_V, _dbl, _D, _D],
+    'gf_species_cp': [_V, _dbl, _D],
+    'gf_species_cv': [_V, _dbl, _D],
+    'gf_enthalpy_mix': [_V, _dbl, _D, _D],
+    'gf_energy_mix': [_V, _dbl, _D, _D],
+    'gf_species_enthalpies': [_V, _dbl, _D],
+    'gf_species_energies': [_V, _dbl, _D],
+    'gf_dcpdT_species': [_V, _dbl, _D, _D],
     'gf_flamelet_stencils': [_V, _D, _int, _D, _D, _D, _D, _D, _D, _D],
     'gf_flamelet_jac_indices': [_V, _int, _I, _I],
     'gf_flamelet_rhs': [_V, _D, _dbl, _D, _D, _int, _D, _D, _D, _D, _int, _D, _D, _D, _D, _D, _D, _int, _int, _int,

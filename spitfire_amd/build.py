@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, 'csrc')
 LIBDIR = os.path.join(HERE, '_lib')
 LIB = os.path.join(LIBDIR, 'libgriffon_amd.so')
 SOURCES = ['gf_twist_4.hip', 'gf_twist_3.hip', 'gf_twist_2.hip', 'gf_twist_1.hip', 'gf_chem.hip', 'gf_btd.hip',
-           'gf_reactor.hip',
+           'gf_reactor.hip', 'gf_reactor_ext.hip',
            'gf_flamelet.hip', 'gf_eig.hip', 'gf_mech.cpp', 'gf_capi.cpp', 'gf_fbatch.cpp']
 ARCH = os.environ.get('GF_OFFLOAD_ARCH', 'gfx950')
 

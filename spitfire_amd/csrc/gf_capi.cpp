@@ -5,6 +5,7 @@
 #include "../../include/griffon_amd.h"
 
 #include <cstring>
+#include <vector>
 
 using namespace gfa;
 
@@ -60,13 +61,78 @@ int prep(gf_mech *m, size_t bytes, Stage &s) {
   return GF_OK;
 }
 
-int check_options(int heat, int open, int rates_opt, int transform_opt) {
-  if (heat != 0 && heat != 1) return fail(GF_ERR_UNSUPPORTED, "diathermal heat transfer is not supported");
-  if (open) return fail(GF_ERR_UNSUPPORTED, "open reactors are not supported");
-  if (rates_opt != 0 && rates_opt != 2)
-    return fail(GF_ERR_UNSUPPORTED, "rates_sensitivity_option 1 (no third-body/falloff sensitivities) is not supported");
+int check_options(const gf_mech *m, int heat, int open, int rates_opt, int transform_opt) {
+  if (!m) return fail(GF_ERR_ARG, "null mechanism");
+  if (heat < 0 || heat > 2) return fail(GF_ERR_ARG, "heat_transfer_option must be 0, 1 or 2");
+  (void)open;
+  if (rates_opt < 0 || rates_opt > 2) return fail(GF_ERR_ARG, "rates_sensitivity_option must be 0, 1 or 2");
+  if (rates_opt == 1)
+    for (const auto &r : m->drx)
+      if (r.hasOrders)
+        return fail(GF_ERR_UNSUPPORTED, "rates_sensitivity_option 1 with special reaction orders is not supported");
   if (transform_opt != 0) return fail(GF_ERR_UNSUPPORTED, "only sensitivity_transform_option 0 exists");
   return GF_OK;
+}
+
+// Reactor RHS (and Jacobian) through the chemistry kernel's per-state outputs and the reactor assembly kernel:
+// open and diathermal terms, option-1 sensitivities and the isochoric reactor.
+int reactor_general(gf_mech *m, bool isochoric, const double *state, double p_or_rho_in, double Tin, const double *yin,
+                    double tau, double Tf, double Ts, double hc, double eps, double sov, int heat, int open,
+                    int rates_opt, double *out_rhs, double *out_jac) {
+  Stage s;
+  const int ns = gf_mech_n_species(m);
+  const int n = isochoric ? ns + 1 : ns;
+  const size_t nsens = (size_t)(ns + 1) * (ns + 1);
+  const bool J = out_jac != nullptr;
+  GF_TRY(prep(m, rup((ns + 1) * 8) + rup(8) + 4 * rup(ns * 8) + rup(4 * 8) + rup(n * 8) + (J ? rup(nsens * 8) + rup((size_t)n * n * 8) : 0), s));
+  double *dst = s.take<double>(ns + 1), *dp = s.take<double>(1), *dw = s.take<double>(ns), *dcp = s.take<double>(ns),
+         *dh = s.take<double>(ns), *dyin = s.take<double>(ns), *daux = s.take<double>(4), *dr = s.take<double>(n);
+  double *dsens = J ? s.take<double>(nsens) : nullptr, *dj = J ? s.take<double>((size_t)n * n) : nullptr;
+  hipStream_t st = m->stream;
+  GF_TRY(h2d(dst, state, n * 8, st));
+  if (!isochoric) GF_TRY(h2d(dp, &p_or_rho_in, 8, st));
+  if (open) GF_TRY(h2d(dyin, yin, ns * 8, st));
+  ChemIO io;
+  io.n = 1;
+  if (isochoric) {  // [rho, T, Y...]: the chemistry reads [T, Y...] with the density given
+    io.x = dst + 1; io.p = dst; io.rho_given = 2;
+  } else {
+    io.x = dst; io.p = dp;
+  }
+  io.xs_i = n; io.xs_v = 1; io.ps = 0;
+  io.wout = dw; io.aux = daux; io.cpi = dcp; io.hsp = dh;
+  io.sens = dsens;
+  io.notbaf = rates_opt == 1;
+  GF_TRY(launch_chem(m, io, st));
+  ReactorExtIO a;
+  a.n = 1; a.isochoric = isochoric; a.state = dst; a.state_stride = n;
+  a.w = dw; a.sens = dsens; a.aux = daux; a.cpi = dcp; a.hsp = dh;
+  a.heat = heat; a.open = open; a.Tin = Tin; a.tau = tau; a.rho_in = isochoric ? p_or_rho_in : 0.; a.yin = dyin;
+  a.Tf = Tf; a.Ts = Ts; a.hc = hc; a.eps = eps; a.sov = sov;
+  a.rhs = dr; a.jac = dj;
+  GF_TRY(launch_reactor_ext(m, a, st));
+  GF_TRY(d2h(out_rhs, dr, n * 8, st));
+  if (J) GF_TRY(d2h(out_jac, dj, (size_t)n * n * 8, st));
+  return sync(st);
+}
+
+// all getters of one (T, y): out = {M, rho(p), p(rho), cp, cv, h, e, dcp/dT, x, cp_i, cv_i, h_i, e_i, dcp_i/dT}
+int thermo_all(gf_mech *m, double T, const double *y, double p, double rho, double *host_out) {
+  Stage s;
+  const int ns = gf_mech_n_species(m);
+  const size_t nout = 8 + 6 * (size_t)ns;
+  GF_TRY(prep(m, rup((ns + 1) * 8) + rup(nout * 8), s));
+  double *din = s.take<double>(ns + 1), *dout = s.take<double>(nout);
+  hipStream_t st = m->stream;
+  GF_TRY(h2d(din, &T, 8, st));
+  if (y) {
+    GF_TRY(h2d(din + 1, y, ns * 8, st));
+  } else if (hipMemsetAsync(din + 1, 0, ns * 8, st) != hipSuccess) {
+    return fail(GF_ERR_HIP, "memset failed");
+  }
+  GF_TRY(launch_thermo_getters(m, din, p, rho, dout, st));
+  GF_TRY(d2h(host_out, dout, nout * 8, st));
+  return sync(st);
 }
 
 }  // namespace
@@ -91,7 +157,7 @@ int gf_production_rates(gf_mech *m, double T, double rho, const double *y, doubl
 
 int gf_prod_rates_primitive_sensitivities(gf_mech *m, double rho, double T, const double *y, int option,
                                           double *out_sens) {
-  GF_TRY(check_options(0, 0, option, 0));
+  GF_TRY(check_options(m, 0, 0, option, 0));
   Stage s;
   const int ns = gf_mech_n_species(m);
   const size_t nsens = (size_t)(ns + 1) * (ns + 1);
@@ -105,6 +171,7 @@ int gf_prod_rates_primitive_sensitivities(gf_mech *m, double rho, double T, cons
   ChemIO io;
   io.n = 1; io.x = dx; io.xs_i = ns + 1; io.xs_v = 1; io.p = dp; io.ps = 0; io.rho_given = 1; io.wout = dw;
   io.sens = dsens;
+  io.notbaf = option == 1;
   GF_TRY(launch_chem(m, io, st));
   GF_TRY(d2h(out_sens, dsens, nsens * 8, st));
   return sync(st);
@@ -113,8 +180,11 @@ int gf_prod_rates_primitive_sensitivities(gf_mech *m, double rho, double T, cons
 int gf_reactor_rhs_isobaric(gf_mech *m, const double *state, double pressure, double inflow_T, const double *inflow_y,
                             double tau, double fluid_T, double surf_T, double h_conv, double eps_rad, double sov,
                             int heat, int open, double *out_rhs) {
-  (void)inflow_T; (void)inflow_y; (void)tau; (void)fluid_T; (void)surf_T; (void)h_conv; (void)eps_rad; (void)sov;
-  GF_TRY(check_options(heat, open, 0, 0));
+  if (!m) return fail(GF_ERR_ARG, "null mechanism");
+  GF_TRY(check_options(m, heat, open, 0, 0));
+  if (open || heat == 2)
+    return reactor_general(m, false, state, pressure, inflow_T, inflow_y, tau, fluid_T, surf_T, h_conv, eps_rad, sov,
+                           heat, open, 0, out_rhs, nullptr);
   Stage s;
   const int ns = gf_mech_n_species(m);
   GF_TRY(prep(m, 2 * rup(ns * 8) + rup(8), s));
@@ -133,8 +203,11 @@ int gf_reactor_rhs_isobaric(gf_mech *m, const double *state, double pressure, do
 int gf_reactor_jac_isobaric(gf_mech *m, const double *state, double pressure, double inflow_T, const double *inflow_y,
                             double tau, double fluid_T, double surf_T, double h_conv, double eps_rad, double sov,
                             int heat, int open, int rates_opt, int transform_opt, double *out_rhs, double *out_jac) {
-  (void)inflow_T; (void)inflow_y; (void)tau; (void)fluid_T; (void)surf_T; (void)h_conv; (void)eps_rad; (void)sov;
-  GF_TRY(check_options(heat, open, rates_opt, transform_opt));
+  if (!m) return fail(GF_ERR_ARG, "null mechanism");
+  GF_TRY(check_options(m, heat, open, rates_opt, transform_opt));
+  if (open || heat == 2 || rates_opt == 1)
+    return reactor_general(m, false, state, pressure, inflow_T, inflow_y, tau, fluid_T, surf_T, h_conv, eps_rad, sov,
+                           heat, open, rates_opt, out_rhs, out_jac);
   Stage s;
   const int ns = gf_mech_n_species(m);
   GF_TRY(prep(m, 2 * rup(ns * 8) + rup(8) + rup((size_t)ns * ns * 8), s));
@@ -155,6 +228,79 @@ int gf_reactor_jac_isobaric(gf_mech *m, const double *state, double pressure, do
   GF_TRY(d2h(out_jac, dj, (size_t)ns * ns * 8, st));
   return sync(st);
 }
+
+int gf_reactor_rhs_isochoric(gf_mech *m, const double *state, double inflow_density, double inflow_T,
+                             const double *inflow_y, double tau, double fluid_T, double surf_T, double h_conv,
+                             double eps_rad, double sov, int heat, int open, double *out_rhs) {
+  if (!m) return fail(GF_ERR_ARG, "null mechanism");
+  GF_TRY(check_options(m, heat, open, 0, 0));
+  return reactor_general(m, true, state, inflow_density, inflow_T, inflow_y, tau, fluid_T, surf_T, h_conv, eps_rad,
+                         sov, heat, open, 0, out_rhs, nullptr);
+}
+
+int gf_reactor_jac_isochoric(gf_mech *m, const double *state, double inflow_density, double inflow_T,
+                             const double *inflow_y, double tau, double fluid_T, double surf_T, double h_conv,
+                             double eps_rad, double sov, int heat, int open, int rates_opt, double *out_rhs,
+                             double *out_jac) {
+  if (!m) return fail(GF_ERR_ARG, "null mechanism");
+  GF_TRY(check_options(m, heat, open, rates_opt, 0));
+  if (!out_jac) return fail(GF_ERR_ARG, "null output array");
+  return reactor_general(m, true, state, inflow_density, inflow_T, inflow_y, tau, fluid_T, surf_T, h_conv, eps_rad,
+                         sov, heat, open, rates_opt, out_rhs, out_jac);
+}
+
+#define GF_THERMO_GETTER(BODY)                                             \
+  if (!m) return fail(GF_ERR_ARG, "null mechanism");                     \
+  const int ns = gf_mech_n_species(m);                                   \
+  std::vector<double> o(8 + 6 * (size_t)ns);                             \
+  BODY
+
+int gf_mixture_molecular_weight(gf_mech *m, const double *y, double *out_value) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, 300., y, 0., 0., o.data())); *out_value = o[0]; return GF_OK;)
+}
+int gf_mole_fractions(gf_mech *m, const double *y, double *out_x) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, 300., y, 0., 0., o.data()));
+                   std::memcpy(out_x, o.data() + 8, ns * 8); return GF_OK;)
+}
+int gf_ideal_gas_density(gf_mech *m, double p, double T, const double *y, double *out_value) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, y, p, 0., o.data())); *out_value = o[1]; return GF_OK;)
+}
+int gf_ideal_gas_pressure(gf_mech *m, double rho, double T, const double *y, double *out_value) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, y, 0., rho, o.data())); *out_value = o[2]; return GF_OK;)
+}
+int gf_cp_mix(gf_mech *m, double T, const double *y, double *out_value) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, y, 0., 0., o.data())); *out_value = o[3]; return GF_OK;)
+}
+int gf_cv_mix(gf_mech *m, double T, const double *y, double *out_value) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, y, 0., 0., o.data())); *out_value = o[4]; return GF_OK;)
+}
+int gf_species_cp(gf_mech *m, double T, double *out_cp) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, nullptr, 0., 0., o.data()));
+                   std::memcpy(out_cp, o.data() + 8 + ns, ns * 8); return GF_OK;)
+}
+int gf_species_cv(gf_mech *m, double T, double *out_cv) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, nullptr, 0., 0., o.data()));
+                   std::memcpy(out_cv, o.data() + 8 + 2 * ns, ns * 8); return GF_OK;)
+}
+int gf_enthalpy_mix(gf_mech *m, double T, const double *y, double *out_value) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, y, 0., 0., o.data())); *out_value = o[5]; return GF_OK;)
+}
+int gf_energy_mix(gf_mech *m, double T, const double *y, double *out_value) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, y, 0., 0., o.data())); *out_value = o[6]; return GF_OK;)
+}
+int gf_species_enthalpies(gf_mech *m, double T, double *out_h) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, nullptr, 0., 0., o.data()));
+                   std::memcpy(out_h, o.data() + 8 + 3 * ns, ns * 8); return GF_OK;)
+}
+int gf_species_energies(gf_mech *m, double T, double *out_e) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, nullptr, 0., 0., o.data()));
+                   std::memcpy(out_e, o.data() + 8 + 4 * ns, ns * 8); return GF_OK;)
+}
+int gf_dcpdT_species(gf_mech *m, double T, const double *y, double *out_dcpdT) {
+  GF_THERMO_GETTER(GF_TRY(thermo_all(m, T, y, 0., 0., o.data()));
+                   std::memcpy(out_dcpdT, o.data() + 8 + 5 * ns, ns * 8); return GF_OK;)
+}
+#undef GF_THERMO_GETTER
 
 int gf_flamelet_stencils(gf_mech *m, const double *dz, int nzi, const double *chi, const double *invLe, double *cmaj,
                          double *csub, double *csup, double *mco, double *nco) {
@@ -267,7 +413,8 @@ int gf_flamelet_jacobian(gf_mech *m, const double *state, double pressure, const
                          int scale_and_offset, double prefactor, int rates_opt, int transform_opt, int incl_ef,
                          int incl_vcp, int scaled_hl, double *out_expeig, double *out_jac) {
   if (nzi < 2) return fail(GF_ERR_ARG, "a flamelet needs at least two interior points");
-  GF_TRY(check_options(0, 0, rates_opt, transform_opt));
+  if (!m) return fail(GF_ERR_ARG, "null mechanism");
+  GF_TRY(check_options(m, 0, 0, rates_opt, transform_opt));
   const int ns = gf_mech_n_species(m);
   const size_t bsz = (size_t)ns * ns, nbl = (size_t)nzi * bsz, noff = (size_t)(nzi - 1) * ns;
   const size_t nel = nbl + 2 * noff;
@@ -284,6 +431,7 @@ int gf_flamelet_jacobian(gf_mech *m, const double *state, double pressure, const
   ChemIO io;
   io.n = nzi; io.x = dpts + ns; io.xs_i = ns; io.xs_v = 1; io.p = dp; io.ps = 0;
   io.rhs = dr; io.rs_i = ns; io.rs_v = 1; io.jac = dj; io.js_i = (long)bsz;
+  io.notbaf = rates_opt == 1;
   GF_TRY(launch_chem(m, io, st));
   if (compute_eigenvalues) {
     // eigenvalues of each chemistry block after the heat-loss terms, before the diffusion terms

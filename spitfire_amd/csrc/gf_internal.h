@@ -136,7 +136,8 @@ struct ChemIO {
   long xs_i = 0, xs_v = 0;
   const double *p = nullptr;
   long ps = 0;
-  int rho_given = 0;
+  int rho_given = 0;       // 1: p holds densities and x [T, y_0..y_{ns-1}]; 2: densities, x [T, y_0..y_{ns-2}]
+  int notbaf = 0;          // sensitivities without third-body / falloff multiplier derivatives (option 1)
   double *rhs = nullptr;   // rhs[i*rs_i + v*rs_v]
   long rs_i = 0, rs_v = 0;
   double *jac = nullptr;   // ns*ns column-major block at jac + i*js_i
@@ -152,6 +153,26 @@ struct ChemIO {
   // optional: the Jacobian block of physical row q starts at jac + js_off[q] (no block if negative)
   const long *js_off = nullptr;
 };
+// Reactor RHS/Jacobian assembly beyond the closed adiabatic isobaric case (gf_reactor_ext.hip), from the chemistry
+// kernel's per-state outputs: w [n][ns], raw sensitivities [n][(ns+1)^2], aux [n][4] = {rho, cp, dcp/dT, M},
+// species cp and h [n][ns]. State s is read at state + s*state_stride: isobaric [T, Y_0..Y_{ns-2}], isochoric
+// [rho, T, Y_0..Y_{ns-2}]. Outputs rhs [n][neq] and jac [n][neq*neq] column-major (neq = ns, or ns+1 isochoric).
+struct ReactorExtIO {
+  int n = 0;
+  int isochoric = 0;
+  const double *state = nullptr;
+  long state_stride = 0;
+  const double *w = nullptr, *sens = nullptr, *aux = nullptr, *cpi = nullptr, *hsp = nullptr;
+  int heat = 0, open = 0;          // heat transfer 0 adiabatic, 1 isothermal, 2 diathermal; open: mass exchange
+  double Tin = 0., tau = 1., rho_in = 0.;
+  const double *yin = nullptr;     // [ns] inflow mass fractions (device)
+  double Tf = 0., Ts = 0., hc = 0., eps = 0., sov = 0.;
+  double *rhs = nullptr, *jac = nullptr;
+};
+int launch_reactor_ext(const gf_mech *m, const ReactorExtIO &a, hipStream_t st);
+// griffon's thermodynamic getters for in = [T, y_0..y_{ns-1}] (device): out = {M, rho(p), p(rho), cp, cv, h, e,
+// dcp/dT, x[ns], cp[ns], cv[ns], h[ns], e[ns], dcp/dT[ns]}
+int launch_thermo_getters(const gf_mech *m, const double *in, double p, double rho, double *out, hipStream_t st);
 int set_chem_launch_shape(int shape);  // -1: timed per launch kind and size class; 0..4: fixed
 int launch_chem(const gf_mech *m, const ChemIO &io, hipStream_t st);
 

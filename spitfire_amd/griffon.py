@@ -282,6 +282,76 @@ class PyCombustionKernels:
                                                 int(sensitivity_transform_option), dptr(rightHandSide),
                                                 dptr(jacobian)))
 
+    def reactor_rhs_isochoric(self, state, inflowDensity, inflowTemperature, inflowY, tau, fluidTemperature,
+                              surfTemperature, hConv, epsRad, surfaceAreaOverVolume, heatTransferOption, open,
+                              rightHandSide):
+        """state [rho, T, Y_0..Y_{ns-2}] (griffon.pyx:831-845)"""
+        for a in (state, inflowY, rightHandSide):
+            _f64(a)
+        check(self._lib.gf_reactor_rhs_isochoric(self._h, dptr(state), float(inflowDensity), float(inflowTemperature),
+                                                 dptr(inflowY), float(tau), float(fluidTemperature),
+                                                 float(surfTemperature), float(hConv), float(epsRad),
+                                                 float(surfaceAreaOverVolume), int(heatTransferOption),
+                                                 int(bool(open)), dptr(rightHandSide)))
+
+    def reactor_jac_isochoric(self, state, inflowDensity, inflowTemperature, inflowY, tau, fluidTemperature,
+                              surfTemperature, hConv, epsRad, surfaceAreaOverVolume, heatTransferOption, open,
+                              rates_sensitivity_option, rightHandSide, jacobian):
+        """(ns+1) x (ns+1) column-major Jacobian in (rho, T, Y) (griffon.pyx:850-866)"""
+        for a in (state, inflowY, rightHandSide, jacobian):
+            _f64(a)
+        check(self._lib.gf_reactor_jac_isochoric(self._h, dptr(state), float(inflowDensity), float(inflowTemperature),
+                                                 dptr(inflowY), float(tau), float(fluidTemperature),
+                                                 float(surfTemperature), float(hConv), float(epsRad),
+                                                 float(surfaceAreaOverVolume), int(heatTransferOption),
+                                                 int(bool(open)), int(rates_sensitivity_option), dptr(rightHandSide),
+                                                 dptr(jacobian)))
+
+    # ---------------- thermodynamics (griffon.pyx:684-760) ----------------
+    def _scalar(self, fn, *args):
+        out = np.zeros(1)
+        check(fn(self._h, *args, dptr(out)))
+        return float(out[0])
+
+    def mixture_molecular_weight(self, y):
+        return self._scalar(self._lib.gf_mixture_molecular_weight, dptr(_f64(y)))
+
+    def mole_fractions(self, y, x):
+        check(self._lib.gf_mole_fractions(self._h, dptr(_f64(y)), dptr(_f64(x))))
+
+    def ideal_gas_density(self, p, T, y):
+        return self._scalar(self._lib.gf_ideal_gas_density, float(p), float(T), dptr(_f64(y)))
+
+    def ideal_gas_pressure(self, rho, T, y):
+        return self._scalar(self._lib.gf_ideal_gas_pressure, float(rho), float(T), dptr(_f64(y)))
+
+    def cp_mix(self, T, y):
+        return self._scalar(self._lib.gf_cp_mix, float(T), dptr(_f64(y)))
+
+    def cv_mix(self, T, y):
+        return self._scalar(self._lib.gf_cv_mix, float(T), dptr(_f64(y)))
+
+    def species_cp(self, T, cpspecies):
+        check(self._lib.gf_species_cp(self._h, float(T), dptr(_f64(cpspecies))))
+
+    def species_cv(self, T, cvspecies):
+        check(self._lib.gf_species_cv(self._h, float(T), dptr(_f64(cvspecies))))
+
+    def enthalpy_mix(self, T, y):
+        return self._scalar(self._lib.gf_enthalpy_mix, float(T), dptr(_f64(y)))
+
+    def energy_mix(self, T, y):
+        return self._scalar(self._lib.gf_energy_mix, float(T), dptr(_f64(y)))
+
+    def species_enthalpies(self, T, enthalpyspecies):
+        check(self._lib.gf_species_enthalpies(self._h, float(T), dptr(_f64(enthalpyspecies))))
+
+    def species_energies(self, T, energyspecies):
+        check(self._lib.gf_species_energies(self._h, float(T), dptr(_f64(energyspecies))))
+
+    def dcpdT_species(self, T, mass_fractions, dcpdT_spec):
+        check(self._lib.gf_dcpdT_species(self._h, float(T), dptr(_f64(mass_fractions)), dptr(_f64(dcpdT_spec))))
+
     # ---------------- batched device entry points (torch tensors on the GPU) ----------------
     def batch_reactor_rhs_isobaric(self, x, p, rhs, layout='soa', stream=None):
         """x, rhs: [ns, n] (layout 'soa') or [n, ns] ('aos') float64 CUDA tensors; p: [n] or [1]."""

@@ -156,10 +156,62 @@ def flamelet_fixture(name, mech, nzi, seed, variants=((0, 0), (0, 1), (1, 0), (1
     print(name, nzi, 'interior points')
 
 
-def main():
+# reactor options exercised by reactors_*.npz: (heat transfer option, open, rates sensitivity option)
+REACTOR_OPTIONS = [(0, 0, 0), (1, 0, 0), (2, 0, 0), (0, 1, 0), (2, 1, 0), (0, 0, 1), (2, 1, 1), (1, 1, 1)]
+REACTOR_PARAMS = dict(Tin=900., tau=1.e-3, Tf=800., Ts=700., hc=50., eps=0.5, sov=30.)
+
+
+def reactor_fixture(name, mech, n_random, seed, max_edge=None):
+    """Isobaric reactors with open / diathermal / isothermal terms and option-1 sensitivities, isochoric reactors
+    with the same options, option-1 raw sensitivities, and the thermodynamic getters, from the reference build."""
+    cm = CompiledMechanism(mech, order='index')
+    ref = oracle.CpuGriffon(cm, ref=True)
+    ns = cm.ns
+    x, p = chem_states(ns, n_random, seed, max_edge)
+    N = x.shape[1]
+    rng = np.random.default_rng(seed + 1000)
+    yin = rng.dirichlet(np.full(ns, 0.5))
+    no = len(REACTOR_OPTIONS)
+    out = dict(seed=mech['seed'], ns=ns, mech_hash=mech_hash(cm), x=x, p=p, yin=yin,
+               options=np.array(REACTOR_OPTIONS), params=np.array([REACTOR_PARAMS[k] for k in
+                                                                   ('Tin', 'tau', 'Tf', 'Ts', 'hc', 'eps', 'sov')]))
+    rho = np.zeros(N)
+    ib_rhs, ib_jac = np.zeros((N, no, ns)), np.zeros((N, no, ns * ns))
+    ic_rhs, ic_jac = np.zeros((N, no, ns + 1)), np.zeros((N, no, (ns + 1) ** 2))
+    ic_rhs_only = np.zeros((N, no, ns + 1))
+    ib_rhs_only = np.zeros((N, no, ns))
+    sens1 = np.zeros((N, (ns + 1) ** 2))
+    thermo_sc, thermo_arr = np.zeros((N, 8)), np.zeros((N, 6, ns))
+    for i in range(N):
+        st = x[:, i]
+        y = np.append(st[1:], 1. - np.sum(st[1:]))
+        mmw = 1. / np.sum(y / cm.mw)
+        rho[i] = p[i] * mmw / (st[0] * cm.gas_constant)
+        sti = np.r_[rho[i], st]
+        for k, (heat, op, ro) in enumerate(REACTOR_OPTIONS):
+            kw = dict(REACTOR_PARAMS, yin=yin, heat=heat, open_=bool(op))
+            ib_rhs[i, k], ib_jac[i, k] = ref.reactor_jac_isobaric_opt(st, p[i], rates_opt=ro, **kw)
+            ib_rhs_only[i, k] = ref.reactor_rhs_isobaric(st, p[i], **kw)
+            ic_rhs[i, k], ic_jac[i, k] = ref.reactor_jac_isochoric(sti, rho_in=1.1 * rho[i], rates_opt=ro, **kw)
+            ic_rhs_only[i, k] = ref.reactor_rhs_isochoric(sti, rho_in=1.1 * rho[i], **kw)
+        sens1[i] = ref.prod_rates_sens_opt(rho[i], st[0], y, 1)
+        sc, arrs = ref.thermo_getters(st[0], y, p[i], rho[i])
+        thermo_sc[i], thermo_arr[i] = sc, np.array(arrs)
+    out.update(rho=rho, ib_rhs=ib_rhs, ib_jac=ib_jac, ib_rhs_only=ib_rhs_only, ic_rhs=ic_rhs, ic_jac=ic_jac,
+               ic_rhs_only=ic_rhs_only, sens1=sens1, thermo_sc=thermo_sc, thermo_arr=thermo_arr)
+    np.savez_compressed(os.path.join(HERE, name), **out)
+    print(name, N, 'states x', no, 'reactor options')
+
+
+def main(which=None):
     oracle.build(ref=True)
     if not oracle.ref_available():
         raise SystemExit('oracle/_ref was not built (needs the reference checkout)')
+    if which in (None, 'reactors'):
+        reactor_fixture('reactors_c1.npz', synthetic.mechanism_c1(), 12, seed=21)
+        reactor_fixture('reactors_53.npz', synthetic.mechanism_53(), 3, seed=22, max_edge=2)
+    if which == 'reactors':
+        return
     chem_fixture('chem_c1.npz', synthetic.mechanism_c1(), 48, seed=11)
     chem_fixture('chem_53.npz', synthetic.mechanism_53(), 12, seed=12, max_edge=8)
     flamelet_fixture('flamelet_c1.npz', synthetic.mechanism_c1(), 12, seed=13)
@@ -167,4 +219,4 @@ def main():
 
 
 if __name__ == '__main__':
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     txt = open(os.path.join(ROOT, 'include', 'griffon_amd.h')).read()
     txt = re.sub(r'/\*.*?\*/', '', txt, flags=re.S)
-    return sorted(set(re.findall(r'\b(gf_[a-z0-9_]+)\s*\(', txt)))
+    return sorted(set(re.findall(r'\b(gf_[A-Za-z0-9_]+)\s*\(', txt)))
 
 
 def test_header_declares_entry_points():

@@ -174,6 +174,20 @@ int gf_batch_reactor_jac_isobaric_terms(gf_mech *m, int n, const double *x, long
                                         long rhs_var_stride, double *jac, long jac_item_stride, double *aux,
                                         double *cpi, double *h, double *qa, void *stream);
 
+/* Reactor RHS and Jacobian over n independent states for every configuration of reactor_rhs/jac_isobaric and
+ * reactor_rhs/jac_isochoric (griffon.pyx:788-866): isochoric 0 reads state i as [T, Y_0..Y_{ns-2}] with pressure
+ * p[i*p_stride]; isochoric 1 reads [rho, T, Y_0..Y_{ns-2}] (p unused, inflow_density is the feed density). State i
+ * starts at x + i*x_item_stride, its variables contiguous. rhs + i*neq and jac + i*neq*neq (column-major; jac may
+ * be null) receive the outputs, neq = ns (+1 isochoric). heat 0/1/2, open, rates_sensitivity_option 0/1/2 and the
+ * feed and heat-transfer parameters as in the drop-ins; inflow_y is a device array of ns mass fractions (used when
+ * open). Device pointers, stream-ordered. Partial results (rates, sensitivities, thermo) go to a workspace the
+ * mechanism owns and grows on demand (synchronous hipMalloc); calls that overlap on one mechanism race on it. */
+int gf_batch_reactor_general(gf_mech *m, int n, int isochoric, const double *x, long x_item_stride, const double *p,
+                             long p_stride, double inflow_density, double inflow_T, const double *inflow_y, double tau,
+                             double fluid_T, double surf_T, double h_conv, double eps_rad,
+                             double surface_area_over_volume, int heat_transfer_option, int open,
+                             int rates_sensitivity_option, double *rhs, double *jac, void *stream);
+
 /* Batched block-tridiagonal (BTDDOD) factor/solve over nsys independent systems of possibly different
  * block counts: system s has nblocks[s] blocks of size bs; its values start at values + voff[s] (griffon
  * layout: nblocks*bs*bs block entries, then (nblocks-1)*bs sub, then (nblocks-1)*bs sup), its L blocks at

@@ -135,6 +135,51 @@ int thermo_all(gf_mech *m, double T, const double *y, double p, double rho, doub
   return sync(st);
 }
 
+// Batched general reactor: chemistry (rates, raw sensitivities, thermo) into `work`, then the reactor assembly.
+// work holds n * batch_work_doubles(ns, jac) doubles.
+size_t batch_work_doubles(int ns, bool jac) {
+  return (size_t)ns + (jac ? (size_t)(ns + 1) * (ns + 1) : 0) + 4 + 2 * (size_t)ns;
+}
+
+int batch_general(const gf_mech *m, int n, int isochoric, const double *x, long x_item_stride, const double *p,
+                  long p_stride, double inflow_density, double inflow_T, const double *inflow_y, double tau,
+                  double fluid_T, double surf_T, double h_conv, double eps_rad, double sov, int heat, int open,
+                  int rates_opt, double *rhs, double *jac, double *work, int no_tune, hipStream_t st) {
+  const int ns = m->view.ns;
+  double *w = work, *cpi = w + (size_t)n * ns, *hsp = cpi + (size_t)n * ns, *aux = hsp + (size_t)n * ns;
+  double *sens = jac ? aux + (size_t)n * 4 : nullptr;
+  ChemIO io;
+  io.n = n;
+  if (isochoric) {
+    io.x = x + 1; io.p = x; io.ps = x_item_stride; io.rho_given = 2;
+  } else {
+    io.x = x; io.p = p; io.ps = p_stride;
+  }
+  io.xs_i = x_item_stride; io.xs_v = 1;
+  io.wout = w; io.aux = aux; io.cpi = cpi; io.hsp = hsp; io.sens = sens;
+  io.notbaf = rates_opt == 1;
+  io.no_tune = no_tune;
+  GF_TRY(launch_chem(m, io, st));
+  ReactorExtIO a;
+  a.n = n; a.isochoric = isochoric; a.state = x; a.state_stride = x_item_stride;
+  a.w = w; a.sens = sens; a.aux = aux; a.cpi = cpi; a.hsp = hsp;
+  a.heat = heat; a.open = open; a.Tin = inflow_T; a.tau = tau; a.rho_in = inflow_density; a.yin = inflow_y;
+  a.Tf = fluid_T; a.Ts = surf_T; a.hc = h_conv; a.eps = eps_rad; a.sov = sov;
+  a.rhs = rhs; a.jac = jac;
+  return launch_reactor_ext(m, a, st);
+}
+
+int check_batch_general(const gf_mech *m, int n, int isochoric, const double *x, long x_item_stride,
+                        const double *p, const double *inflow_y, int heat, int open, int rates_opt, double *rhs) {
+  if (!m) return fail(GF_ERR_ARG, "null mechanism");
+  if (n < 0) return fail(GF_ERR_ARG, "negative number of states");
+  if (n > 0 && (!x || !rhs || (!isochoric && !p) || (open && !inflow_y)))
+    return fail(GF_ERR_ARG, "null device array");
+  const int neq = m->view.ns + (isochoric ? 1 : 0);
+  if (x_item_stride < neq) return fail(GF_ERR_ARG, "state item stride is smaller than the state size");
+  return check_options(m, heat, open, rates_opt, 0);
+}
+
 }  // namespace
 
 extern "C" {
@@ -560,6 +605,19 @@ int gf_batch_reactor_jac_isobaric_terms(gf_mech *m, int n, const double *x, long
   io.n = n; io.x = x; io.xs_i = xs_i; io.xs_v = xs_v; io.p = p; io.ps = ps; io.rhs = rhs; io.rs_i = rs_i; io.rs_v = rs_v;
   io.jac = jac; io.js_i = js_i; io.aux = aux; io.cpi = cpi; io.hsp = h; io.qa = qa;
   return launch_chem(m, io, static_cast<hipStream_t>(stream));
+}
+
+int gf_batch_reactor_general(gf_mech *m, int n, int isochoric, const double *x, long x_item_stride, const double *p,
+                             long p_stride, double inflow_density, double inflow_T, const double *inflow_y, double tau,
+                             double fluid_T, double surf_T, double h_conv, double eps_rad, double sov, int heat,
+                             int open, int rates_opt, double *rhs, double *jac, void *stream) {
+  GF_TRY(check_batch_general(m, n, isochoric, x, x_item_stride, p, inflow_y, heat, open, rates_opt, rhs));
+  if (n == 0) return GF_OK;
+  GF_TRY(ensure_device(m));
+  GF_TRY(ensure_work(m, (size_t)n * batch_work_doubles(m->view.ns, jac != nullptr) * sizeof(double)));
+  return batch_general(m, n, isochoric, x, x_item_stride, p, p_stride, inflow_density, inflow_T, inflow_y, tau,
+                       fluid_T, surf_T, h_conv, eps_rad, sov, heat, open, rates_opt, rhs, jac,
+                       static_cast<double *>(m->work), 0, static_cast<hipStream_t>(stream));
 }
 
 int gf_batch_btddod_full_factorize(int nsys, const int *nblocks, int bs, double *values, const long *voff,

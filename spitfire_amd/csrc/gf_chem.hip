@@ -716,6 +716,7 @@ int launch_chem(const gf_mech *m, const ChemIO &a, hipStream_t st) {
   }
   const int k = 63 - __builtin_clzll((unsigned long long)a.n);
   signed char &slot = m->chem_shape[J ? 0 : 1][k];
+  if (a.no_tune) return launch_shape(m, a, st, fallback(slot >= 0 ? slot : 2), J);
   if (slot < 0) {
     int best = 2;
     const int rc = autotune_shape(m, a, st, J, best);

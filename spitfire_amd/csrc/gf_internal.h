@@ -119,6 +119,9 @@ struct gf_mech {
   // staging buffers for the host-pointer drop-ins
   void *stage = nullptr;
   size_t stage_bytes = 0;
+  // device workspace of the batched entry points that pass partial results between kernels
+  void *work = nullptr;
+  size_t work_bytes = 0;
   hipStream_t stream = nullptr;
 };
 
@@ -127,6 +130,7 @@ void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 int ensure_device(gf_mech *m);                 // finalize + upload on first compute call
 int ensure_stage(gf_mech *m, size_t bytes);
+int ensure_work(gf_mech *m, size_t bytes);   // grows m->work (synchronous hipFree/hipMalloc)
 
 // Inputs/outputs of one batched chemistry launch (gf_chem.hip). Element v of state i is x[i*xs_i + v*xs_v];
 // p[i*ps] is its pressure (or its density when rho_given, in which case x holds [T, y_0..y_{ns-1}]).
@@ -138,6 +142,7 @@ struct ChemIO {
   long ps = 0;
   int rho_given = 0;       // 1: p holds densities and x [T, y_0..y_{ns-1}]; 2: densities, x [T, y_0..y_{ns-2}]
   int notbaf = 0;          // sensitivities without third-body / falloff multiplier derivatives (option 1)
+  int no_tune = 0;         // never time launch shapes: use the tuned shape if there is one, else the default
   double *rhs = nullptr;   // rhs[i*rs_i + v*rs_v]
   long rs_i = 0, rs_v = 0;
   double *jac = nullptr;   // ns*ns column-major block at jac + i*js_i

@@ -300,6 +300,16 @@ int ensure_stage(gf_mech *m, size_t bytes) {
   return GF_OK;
 }
 
+int ensure_work(gf_mech *m, size_t bytes) {
+  if (m->work_bytes >= bytes) return GF_OK;
+  if (m->work) (void)hipFree(m->work);
+  m->work = nullptr;
+  m->work_bytes = 0;
+  if (hipMalloc(&m->work, bytes) != hipSuccess) return fail(GF_ERR_HIP, "hipMalloc of the workspace failed");
+  m->work_bytes = bytes;
+  return GF_OK;
+}
+
 }  // namespace gfa
 
 using namespace gfa;
@@ -324,6 +334,7 @@ int gf_mech_destroy(gf_mech *m) {
   if (!m) return GF_OK;
   if (m->dblob) (void)hipFree(m->dblob);
   if (m->stage) (void)hipFree(m->stage);
+  if (m->work) (void)hipFree(m->work);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return GF_OK;

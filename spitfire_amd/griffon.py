@@ -361,6 +361,35 @@ class PyCombustionKernels:
                                                       1 if p.numel() > 1 else 0, rhs.data_ptr(), rs[0], rs[1],
                                                       _stream(stream)))
 
+    def batch_reactor_general(self, x, rhs, jac=None, p=None, isochoric=False, inflow_density=0., inflow_T=0.,
+                              inflow_y=None, tau=1., fluid_T=0., surf_T=0., h_conv=0., eps_rad=0.,
+                              surface_area_over_volume=0., heat_transfer_option=0, open=False,
+                              rates_sensitivity_option=0, stream=None):
+        """Any reactor configuration over n states (gf_batch_reactor_general). x: [n, neq] float64 CUDA tensor
+        ([T, Y..] isobaric with p: [n] or [1] pressures; [rho, T, Y..] isochoric); rhs: [n, neq]; jac: None or
+        [n, neq*neq] (column-major blocks); inflow_y: [ns] CUDA tensor when open."""
+        n, neq = self._general_shapes(x, rhs, jac, isochoric)
+        if not isochoric and (p is None or not p.is_cuda):
+            raise ValueError('isobaric reactors need a CUDA pressure tensor p')
+        check(self._lib.gf_batch_reactor_general(
+            self._h, n, int(bool(isochoric)), x.data_ptr(), neq, p.data_ptr() if p is not None else None,
+            (1 if p.numel() > 1 else 0) if p is not None else 0, float(inflow_density), float(inflow_T),
+            inflow_y.data_ptr() if inflow_y is not None else None, float(tau), float(fluid_T), float(surf_T),
+            float(h_conv), float(eps_rad), float(surface_area_over_volume), int(heat_transfer_option),
+            int(bool(open)), int(rates_sensitivity_option), rhs.data_ptr(), jac.data_ptr() if jac is not None else None,
+            _stream(stream)))
+
+    def _general_shapes(self, x, rhs, jac, isochoric):
+        import torch
+        neq = self._ns + (1 if isochoric else 0)
+        for t in (x, rhs) + ((jac,) if jac is not None else ()):
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or t.dim() != 2:
+                raise ValueError('expected contiguous 2-D float64 CUDA tensors')
+        n = x.shape[0]
+        if x.shape[1] != neq or tuple(rhs.shape) != (n, neq) or (jac is not None and tuple(jac.shape) != (n, neq * neq)):
+            raise ValueError('x and rhs must be [n, %d] and jac [n, %d]' % (neq, neq * neq))
+        return n, neq
+
     def batch_reactor_jac_isobaric(self, x, p, rhs, jac, layout='soa', stream=None):
         """As batch_reactor_rhs_isobaric; jac: [n, ns*ns] float64 (column-major ns x ns block per state)."""
         n, xs = _strides(x, self._ns, layout)

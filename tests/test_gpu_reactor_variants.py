@@ -95,3 +95,48 @@ def test_option_one_with_special_orders_is_rejected():
     spec.griffon.prod_rates_primitive_sensitivities(0.5, 1200., y, 0, out)
     with pytest.raises(NativeError):
         spec.griffon.prod_rates_primitive_sensitivities(0.5, 1200., y, 1, out)
+
+
+@pytest.mark.parametrize('isochoric', [False, True])
+def test_batched_general_reactor_matches_dropins(isochoric, mech_53_data):
+    """gf_batch_reactor_general over many states equals the single-state drop-ins bitwise (the same kernels)."""
+    import torch
+    from spitfire_amd import synthetic
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    spec = ChemicalMechanismSpec(mech_53_data)
+    g, ns = spec.griffon, spec.n_species
+    x, p, _ = synthetic.random_states(ns, 40, seed=5)
+    X = np.ascontiguousarray(x.T)
+    mw = spec.molecular_weights
+    Y = np.hstack([X[:, 1:], 1. - X[:, 1:].sum(axis=1, keepdims=True)])
+    rho = p / (spec.gas_constant * X[:, 0]) / np.sum(Y / mw, axis=1)
+    if isochoric:
+        X = np.ascontiguousarray(np.hstack([rho[:, None], X]))
+    neq = X.shape[1]
+    yin = np.random.default_rng(3).dirichlet(np.full(ns, 0.5))
+    dev = torch.device('cuda:0')
+    kw = dict(inflow_density=0.9, inflow_T=700., tau=2e-3, fluid_T=600., surf_T=650., h_conv=20., eps_rad=0.3,
+              surface_area_over_volume=12.)
+    for heat, op, ro in [(0, False, 0), (2, True, 0), (1, True, 1)]:
+        print('batched general reactor: isochoric', isochoric, 'options', heat, op, ro, flush=True)
+        xt, rt = torch.tensor(X, device=dev), torch.zeros((len(X), neq), dtype=torch.float64, device=dev)
+        jt = torch.zeros((len(X), neq * neq), dtype=torch.float64, device=dev)
+        g.batch_reactor_general(xt, rt, jt, p=torch.tensor(p, device=dev), isochoric=isochoric,
+                                inflow_y=torch.tensor(yin, device=dev), heat_transfer_option=heat, open=op,
+                                rates_sensitivity_option=ro, **kw)
+        torch.cuda.synchronize()
+        R, J = rt.cpu().numpy(), jt.cpu().numpy()
+        for i in (0, 17, 39):
+            r, j = np.zeros(neq), np.zeros(neq * neq)
+            args = (kw['inflow_T'], yin, kw['tau'], kw['fluid_T'], kw['surf_T'], kw['h_conv'], kw['eps_rad'],
+                    kw['surface_area_over_volume'], heat, op, ro)
+            if isochoric:
+                g.reactor_jac_isochoric(np.ascontiguousarray(X[i]), kw['inflow_density'], *args, r, j)
+            else:
+                g.reactor_jac_isobaric(np.ascontiguousarray(X[i]), float(p[i]), *args, 0, r, j)
+            if not isochoric and (heat, op, ro) == (0, False, 0):
+                # the closed adiabatic isobaric drop-in takes the fused chemistry path: same values, other rounding
+                assert _vec(R[i], r) <= RTOL and _rows(J[i], j, neq) <= RTOL
+            else:
+                assert np.array_equal(R[i], r) and np.array_equal(J[i], j), (i, np.max(np.abs(R[i] - r)),
+                                                                              np.max(np.abs(J[i] - j)))

@@ -188,6 +188,20 @@ int gf_batch_reactor_general(gf_mech *m, int n, int isochoric, const double *x, 
                              double surface_area_over_volume, int heat_transfer_option, int open,
                              int rates_sensitivity_option, double *rhs, double *jac, void *stream);
 
+/* gf_batch_reactor_general without any allocation (no counterpart in the reference). It is for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. The
+ * partial results live in a fixed 32 MiB device array of the library (one per GPU, held until the process ends)
+ * instead of the mechanism's workspace: n may be at most floor(4194304 / (3 ns + 4 + (ns+1)^2)) states with a
+ * Jacobian (1,362 at 53 species) or floor(4194304 / (3 ns + 4)) without; larger n returns GF_ERR_ARG and launches
+ * nothing. It calls no hipMalloc/hipFree and never times launch shapes; the mechanism must already be resident on
+ * the device (any earlier compute call on it), else GF_ERR_STATE. Every call on one GPU shares that array, so calls
+ * that overlap on one GPU race; results are bitwise those of gf_batch_reactor_general. */
+int gf_batch_reactor_general_noalloc(gf_mech *m, int n, int isochoric, const double *x, long x_item_stride,
+                                     const double *p, long p_stride, double inflow_density, double inflow_T,
+                                     const double *inflow_y, double tau, double fluid_T, double surf_T, double h_conv,
+                                     double eps_rad, double surface_area_over_volume, int heat_transfer_option,
+                                     int open, int rates_sensitivity_option, double *rhs, double *jac, void *stream);
+
 /* Batched block-tridiagonal (BTDDOD) factor/solve over nsys independent systems of possibly different
  * block counts: system s has nblocks[s] blocks of size bs; its values start at values + voff[s] (griffon
  * layout: nblocks*bs*bs block entries, then (nblocks-1)*bs sub, then (nblocks-1)*bs sup), its L blocks at

@@ -71,6 +71,8 @@ SIGNATURES = {
     'gf_batch_reactor_jac_isobaric': [_V, _int, _V, _long, _long, _V, _long, _V, _long, _long, _V, _long, _V],
     'gf_batch_reactor_general': [_V, _int, _int, _V, _long, _V, _long, _dbl, _dbl, _V, _dbl, _dbl, _dbl, _dbl,
                                  _dbl, _dbl, _int, _int, _int, _V, _V, _V],
+    'gf_batch_reactor_general_noalloc': [_V, _int, _int, _V, _long, _V, _long, _dbl, _dbl, _V, _dbl, _dbl, _dbl,
+                                         _dbl, _dbl, _dbl, _int, _int, _int, _V, _V, _V],
     'gf_batch_btddod_full_factorize': [_int, _V, _int, _V, _V, _V, _V, _V, _V, _V],
     'gf_batch_btddod_full_solve': [_int, _V, _int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V],
     'gf_batch_reactor_jac_isobaric_terms': [_V, _int, _V, _long, _long, _V, _long, _V, _long, _long, _V, _long, _V, _V, _V, _V, _V],

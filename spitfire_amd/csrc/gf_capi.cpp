@@ -135,6 +135,10 @@ int thermo_all(gf_mech *m, double T, const double *y, double p, double rho, doub
   return sync(st);
 }
 
+}  // namespace
+
+namespace gfa {
+
 // Batched general reactor: chemistry (rates, raw sensitivities, thermo) into `work`, then the reactor assembly.
 // work holds n * batch_work_doubles(ns, jac) doubles.
 size_t batch_work_doubles(int ns, bool jac) {
@@ -180,7 +184,7 @@ int check_batch_general(const gf_mech *m, int n, int isochoric, const double *x,
   return check_options(m, heat, open, rates_opt, 0);
 }
 
-}  // namespace
+}  // namespace gfa
 
 extern "C" {
 

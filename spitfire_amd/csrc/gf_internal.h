@@ -175,6 +175,15 @@ struct ReactorExtIO {
   double *rhs = nullptr, *jac = nullptr;
 };
 int launch_reactor_ext(const gf_mech *m, const ReactorExtIO &a, hipStream_t st);
+// gf_batch_reactor_general's work (gf_capi.cpp): doubles of workspace per state, argument checks, and the two
+// launches on a given workspace
+size_t batch_work_doubles(int ns, bool jac);
+int check_batch_general(const gf_mech *m, int n, int isochoric, const double *x, long x_item_stride,
+                        const double *p, const double *inflow_y, int heat, int open, int rates_opt, double *rhs);
+int batch_general(const gf_mech *m, int n, int isochoric, const double *x, long x_item_stride, const double *p,
+                  long p_stride, double inflow_density, double inflow_T, const double *inflow_y, double tau,
+                  double fluid_T, double surf_T, double h_conv, double eps_rad, double sov, int heat, int open,
+                  int rates_opt, double *rhs, double *jac, double *work, int no_tune, hipStream_t st);
 // griffon's thermodynamic getters for in = [T, y_0..y_{ns-1}] (device): out = {M, rho(p), p(rho), cp, cv, h, e,
 // dcp/dT, x[ns], cp[ns], cv[ns], h[ns], e[ns], dcp/dT[ns]}
 int launch_thermo_getters(const gf_mech *m, const double *in, double p, double rho, double *out, hipStream_t st);

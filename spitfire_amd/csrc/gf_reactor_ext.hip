@@ -16,6 +16,8 @@
 #include "gf_thermo.h"
 #include "../../include/griffon_amd.h"
 
+#include <mutex>
+
 namespace gfa {
 
 namespace {
@@ -259,4 +261,60 @@ int launch_thermo_getters(const gf_mech *m, const double *in, double p, double r
   return GF_OK;
 }
 
+// ---------------- allocation-free variant of gf_batch_reactor_general ----------------
+// Its workspace is this file-scope device array: each GPU holds its own copy for the life of the process. Host code
+// never passes `scratch` itself to a kernel: its device address on the current GPU comes from hipGetSymbolAddress,
+// looked up once per device on the first call and cached.
+namespace {
+constexpr size_t kScratchBytes = size_t(32) << 20;  // 32 MiB per device
+constexpr size_t kScratchDoubles = kScratchBytes / sizeof(double);
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double scratch[kScratchDoubles];
+double *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+int scratch_on_current_device(double **ptr) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(GF_ERR_HIP, "no HIP device available");
+  if (dev < 0 || dev >= kMaxDevices) return fail(GF_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(GF_ERR_HIP, std::string("hipGetSymbolAddress of the scratch array failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return GF_OK;
+}
+}  // namespace
+
 }  // namespace gfa
+
+extern "C" int gf_batch_reactor_general_noalloc(gf_mech *m, int n, int isochoric, const double *x, long x_item_stride,
+                                                const double *p, long p_stride, double inflow_density,
+                                                double inflow_T, const double *inflow_y, double tau, double fluid_T,
+                                                double surf_T, double h_conv, double eps_rad, double sov, int heat,
+                                                int open, int rates_opt, double *rhs, double *jac, void *stream) {
+  using namespace gfa;
+  const int rc = check_batch_general(m, n, isochoric, x, x_item_stride, p, inflow_y, heat, open, rates_opt, rhs);
+  if (rc != GF_OK) return rc;
+  if (n == 0) return GF_OK;
+  if (!m->finalized || !m->dblob)
+    return fail(GF_ERR_STATE, "the mechanism is not on the device yet; make one call through another entry point "
+                              "first (this entry point allocates nothing, the mechanism tables included)");
+  const size_t per_state = batch_work_doubles(m->view.ns, jac != nullptr);
+  const size_t max_n = kScratchDoubles / per_state;
+  if ((size_t)n > max_n)
+    return fail(GF_ERR_ARG, "n = " + std::to_string(n) + " exceeds the allocation-free scratch (" +
+                                std::to_string(max_n) + " states of this mechanism " +
+                                (jac ? "with" : "without") + " Jacobians fit in 32 MiB)");
+  double *work = nullptr;
+  const int r2 = scratch_on_current_device(&work);
+  if (r2 != GF_OK) return r2;
+  return batch_general(m, n, isochoric, x, x_item_stride, p, p_stride, inflow_density, inflow_T, inflow_y, tau,
+                       fluid_T, surf_T, h_conv, eps_rad, sov, heat, open, rates_opt, rhs, jac, work, 1,
+                       static_cast<hipStream_t>(stream));
+}
+

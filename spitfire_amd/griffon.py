@@ -368,10 +368,27 @@ class PyCombustionKernels:
         """Any reactor configuration over n states (gf_batch_reactor_general). x: [n, neq] float64 CUDA tensor
         ([T, Y..] isobaric with p: [n] or [1] pressures; [rho, T, Y..] isochoric); rhs: [n, neq]; jac: None or
         [n, neq*neq] (column-major blocks); inflow_y: [ns] CUDA tensor when open."""
+        self._general(self._lib.gf_batch_reactor_general, x, rhs, jac, p, isochoric, inflow_density, inflow_T,
+                      inflow_y, tau, fluid_T, surf_T, h_conv, eps_rad, surface_area_over_volume, heat_transfer_option,
+                      open, rates_sensitivity_option, stream)
+
+    def batch_reactor_general_noalloc(self, x, rhs, jac=None, p=None, isochoric=False, inflow_density=0.,
+                                      inflow_T=0., inflow_y=None, tau=1., fluid_T=0., surf_T=0., h_conv=0.,
+                                      eps_rad=0., surface_area_over_volume=0., heat_transfer_option=0, open=False,
+                                      rates_sensitivity_option=0, stream=None):
+        """batch_reactor_general on the library's fixed device scratch: no allocation, bounded n, calls that overlap
+        on one GPU race (gf_batch_reactor_general_noalloc)."""
+        self._general(self._lib.gf_batch_reactor_general_noalloc, x, rhs, jac, p, isochoric, inflow_density,
+                      inflow_T, inflow_y, tau, fluid_T, surf_T, h_conv, eps_rad, surface_area_over_volume,
+                      heat_transfer_option, open, rates_sensitivity_option, stream)
+
+    def _general(self, fn, x, rhs, jac, p, isochoric, inflow_density, inflow_T, inflow_y, tau, fluid_T, surf_T,
+                 h_conv, eps_rad, surface_area_over_volume, heat_transfer_option, open, rates_sensitivity_option,
+                 stream):
         n, neq = self._general_shapes(x, rhs, jac, isochoric)
         if not isochoric and (p is None or not p.is_cuda):
             raise ValueError('isobaric reactors need a CUDA pressure tensor p')
-        check(self._lib.gf_batch_reactor_general(
+        check(fn(
             self._h, n, int(bool(isochoric)), x.data_ptr(), neq, p.data_ptr() if p is not None else None,
             (1 if p.numel() > 1 else 0) if p is not None else 0, float(inflow_density), float(inflow_T),
             inflow_y.data_ptr() if inflow_y is not None else None, float(tau), float(fluid_T), float(surf_T),

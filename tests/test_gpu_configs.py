@@ -1,0 +1,160 @@
+"""The flamelet and adjoint workloads at their full sizes on the GPU, each against the oracle.
+
+- One steady flamelet on the 53-species mechanism, 128-point grid clustered at Z_st, chi_st = 10 1/s, air against
+  H2:N2 1:1 (mass) at 300 K and 1 atm: the batched pseudo-transient solver with FP64 and with FP32 block factors
+  against the reference algorithm on the oracle (OracleFlamelet.steady_solve_psitc), all to a scaled residual of
+  1e-8; converged states agree to 1e-8 scaled (T / 1e3, Y).
+- Seven flamelets of the seed-0 library of 512 (grid sizes uniform in 64..256, chi_st log-uniform in 1e-2..1e3):
+  both grid-size extremes (64 and 256 points), the smallest dissipation rate (chi_st = 0.010), dissipation rates
+  near 1e3 and three extinguished flamelets, solved together by the batched steady-state cascade (Newton, then
+  pseudo-transient continuation with the reference's 20 restarts) with FP64 and FP32 block factors; each against
+  the oracle's pseudo-transient solve: the same outcome, the same extinguished/burning classification
+  (max(T - T_linear) < 10 K, tabulation.py:292) and states equal to 1e-8 scaled.
+- 1024 isobaric H2/air reactors on the 53-species mechanism (T0 ~ U[1000, 1600] K, phi ~ U[0.5, 2], 1 atm, seed 0)
+  integrated to 2 ms: the discrete-adjoint gradient of the integrated heat release with respect to ln A against
+  batched central differences of the frozen-step forward map on six reactions (two of each type; all 1024 reactors;
+  step 1e-3, stage equations to 1e-14; bound 1e-4 normwise per reactor: through the ignitions a smaller step is
+  swamped by the Newton tolerance carried through the explosive growth), and, for two reactors, against the
+  tangent-linear (forward-sensitivity) gradient of the oracle's own integration of those reactors (all 325
+  reactions, bound 1e-6 normwise per reactor).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LIBRARY_PICKS = [53, 13, 27, 42, 284, 23, 93]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _streams(m):
+    return m.stream(stp_air=True), m.stream('TPY', (300., 101325., 'H2:1 N2:1'))
+
+
+def _flamelet(cls, m, chi, npts):
+    from spitfire_amd.flamelet import FlameletSpec
+    air, fuel = _streams(m)
+    return cls(FlameletSpec(mech_spec=m, oxy_stream=air, fuel_stream=fuel, grid_points=int(npts),
+                            stoich_dissipation_rate=float(chi)))
+
+
+def _scaled(a, b, ns):
+    return float(np.max(np.abs(a - b) / np.tile(np.r_[1e3, np.ones(ns - 1)], a.size // ns)))
+
+
+def _library_draws(n=512, seed=0):
+    rng = np.random.default_rng(seed)
+    npts = rng.integers(64, 257, n)
+    chis = 10. ** rng.uniform(-2., 3., n)
+    return npts, chis
+
+
+def test_one_flamelet_128_points_fp64_fp32_vs_oracle(mech_53_data, oracle_lib):
+    from spitfire_amd.flamelet import Flamelet, FlameletBatch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from oracle.flamelet_oracle import OracleFlamelet
+    m = ChemicalMechanismSpec(mech_53_data)
+    ns = m.n_species
+    say('one flamelet, 128 points, chi_st 10: oracle psitc')
+    ref, it, ok, _ = _flamelet(OracleFlamelet, m, 10., 128).steady_solve_psitc(tolerance=1e-8)
+    assert ok
+    for prec in (64, 32):
+        fb = FlameletBatch([_flamelet(Flamelet, m, 10., 128)])
+        states, conv, its = fb.steady_solve_psitc(tolerance=1e-8, lu_precision=prec)
+        e = _scaled(states[0], ref, ns)
+        say('  fp%d: converged %s, %d iterations (oracle %d), max scaled diff to the oracle %.3e, residual %.2e'
+            % (prec, bool(conv[0]), its[0], it, e, fb.residuals[0]))
+        assert conv[0] and e <= 1e-8
+
+
+def test_library_flamelets_cascade_fp64_fp32_vs_oracle(mech_53_data, oracle_lib):
+    from spitfire_amd.flamelet import Flamelet, FlameletBatch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from oracle.flamelet_oracle import OracleFlamelet
+    m = ChemicalMechanismSpec(mech_53_data)
+    ns = m.n_species
+    npts, chis = _library_draws()
+    pick = LIBRARY_PICKS
+    assert min(npts[pick]) == 64 and max(npts[pick]) == 256 and min(chis[pick]) < 0.0101 and max(chis[pick]) > 900.
+    refs = []
+    for k in pick:
+        f = _flamelet(OracleFlamelet, m, chis[k], npts[k])
+        st, it, ok, _ = f.steady_solve_psitc(tolerance=1e-8)
+        ext = ok and np.max(st[::ns] - f.linear_temperature[1:-1]) < 10.
+        say('oracle: flamelet %d, %d points, chi_st %.4g: converged %s in %s iterations, extinguished %s'
+            % (k, npts[k], chis[k], ok, it, ext))
+        refs.append((st, ok, ext))
+    for prec in (64, 32):
+        fl = [_flamelet(Flamelet, m, chis[k], npts[k]) for k in pick]
+        fb = FlameletBatch(fl)
+        states, conv, its = fb.compute_steady_state(tolerance=1e-8, lu_precision=prec, raise_on_failure=False,
+                                                    use_time_stepping=False)
+        for j, k in enumerate(pick):
+            st, ok, ext = refs[j]
+            assert bool(conv[j]) == ok, (k, prec)
+            if ok:
+                e = _scaled(states[j], st, ns)
+                gext = np.max(states[j][::ns] - fl[j].linear_temperature[1:-1]) < 10.
+                say('  fp%d flamelet %d: stage %s, %d iterations, extinguished %s, max scaled diff %.3e'
+                    % (prec, k, fb.method[j], its[j], gext, e))
+                assert gext == ext and e <= 1e-8
+
+
+def _adjoint_reactors(m, n, seed=0):
+    rng = np.random.default_rng(seed)
+    T0 = rng.uniform(1000., 1600., n)
+    phi = rng.uniform(0.5, 2.0, n)
+    names = list(m.species_names)
+    X = np.zeros((n, m.n_species))
+    X[:, names.index('H2')], X[:, names.index('O2')], X[:, names.index('N2')] = 2. * phi, 1., 3.76
+    Y = X * m.molecular_weights[None, :]
+    return T0, Y / Y.sum(axis=1, keepdims=True)
+
+
+def test_adjoint_1024_reactors_vs_differences_and_tangent_linear(mech_53_data, oracle_lib):
+    import torch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from spitfire_amd.reactor_batch import ReactorBatch
+    from oracle.reactor_batch_oracle import OracleReactorBatch, scaled_mechanism, tangent_linear_heat_release_gradient
+    m = ChemicalMechanismSpec(mech_53_data)
+    T0, Y0 = _adjoint_reactors(m, 1024)
+    t_end = 2.e-3
+    say('1024 reactors to 2 ms: forward and adjoint')
+    g = ReactorBatch(m, T0, Y0)
+    g.integrate(t_end)
+    L, G = g.heat_release_gradient()
+    G = G.cpu().numpy()
+    assert np.all(np.isfinite(G))
+    sched = g.step_sizes()
+    types = [rx['type'] for rx in m.mech_data['reactions']]
+    order = np.argsort(-np.abs(G).sum(axis=0))
+    pick = []
+    for t in ('simple', 'three-body', 'Troe'):
+        pick += [int(r) for r in order if types[r] == t][:2]
+    eps = 1.e-3
+    FD = np.zeros((len(T0), len(pick)))
+    for c, r in enumerate(pick):
+        Ls = []
+        for sgn in (1., -1.):
+            say('  frozen-step integration, reaction %d (%s), ln A %+g' % (r, types[r], sgn * eps))
+            b = ReactorBatch(ChemicalMechanismSpec(scaled_mechanism(m.mech_data, r, np.exp(sgn * eps))), T0, Y0)
+            b.integrate(t_end, dt_schedule=sched, nonlinear_solve_tolerance=1.e-14)
+            Ls.append(b.heat_release().cpu().numpy())
+            del b
+            torch.cuda.empty_cache()
+        FD[:, c] = (Ls[0] - Ls[1]) / (2. * eps)
+    err = np.max(np.abs(G[:, pick] - FD), axis=1) / np.max(np.abs(FD), axis=1)
+    say('  steps %d, reactions %s, normwise adjoint vs differences: max over reactors %.3e, median %.3e'
+        % (g.stats['steps'], pick, err.max(), np.median(err)))
+    assert err.max() <= 1e-4
+    sel = [0, 511]
+    say('  tangent-linear reference on the oracle for reactors', sel)
+    o = OracleReactorBatch(m, T0[sel], Y0[sel])
+    o.integrate(t_end)
+    Tl = tangent_linear_heat_release_gradient(o).numpy()
+    e_tl = np.max(np.abs(G[sel] - Tl), axis=1) / np.max(np.abs(Tl), axis=1)
+    say('  normwise GPU adjoint vs oracle tangent-linear:', e_tl)
+    assert np.all(e_tl <= 1e-6)

@@ -403,8 +403,12 @@ double mass_action_deriv(const std::vector<int> &idx, const std::vector<int> &st
 // notbaf: rates_sensitivities_no_tbaf_sens.cpp (option 1), the same rates with the derivatives of the third-body
 // and falloff multiplier left out: dq = dRnet * Ctbaf (no_tbaf_sens.cpp:788-795). Special orders are not restated for
 // option 1 (that file has no special-order branch).
+// qa (optional, [nr] in griffon's reaction order) receives d(rate of progress q_r)/d(ln A_r) with A_r the forward
+// (high-pressure, for falloff) pre-exponential factor: kf and kr = kf/Kc both scale with A_r, so for simple and
+// three-body reactions it is q_r = Rnet * Ctbaf; for falloff, Pr = k0 [M] / kinf also moves, giving
+// Rnet * (Ctbaf - Pr dCtbaf/dPr), with Pr dCtbaf/dPr = [M] * (k0 / kinf) dCtbaf/dPr (the nsTmp factor below).
 void prod_rates_sens_exact(const Mech &m, double T, double rho, double Mmix, const double *y, double *w, double *ws,
-                           bool notbaf = false) {
+                           bool notbaf = false, double *qa = nullptr) {
   const int ns = m.ns;
   const int nsp1 = ns + 1;
   for (int i = 0; i < ns; ++i) w[i] = 0.;
@@ -419,9 +423,11 @@ void prod_rates_sens_exact(const Mech &m, double T, double rho, double Mmix, con
   const double *invMsp = m.invmw.data();
   double Kc = 0., dKcdToverKc = 0.;  // persist across reactions as in the reference (n_net > 6 keeps the previous value)
 
+  int ridx = -1;
   for (const Rxn &r : m.rx) {
+    ++ridx;
     double Rnet = 0., dRdrho = 0., dRdT = 0.;
-    double Ctbaf = 1., dCdrho = 0., dCdT = 0.;
+    double Ctbaf = 1., dCdrho = 0., dCdT = 0., lnkinf_factor = 1.;
     for (int i = 0; i < ns - 1; ++i) { dRdY[i] = 0.; dCdY[i] = 0.; dqdY[i] = 0.; }
     const double kf = kf_of(r, T, logT, invT);
 
@@ -557,6 +563,7 @@ void prod_rates_sens_exact(const Mech &m, double T, double rho, double Mmix, con
         dCdT = 1. / (1. + 1. / pr) * dfTdT + fT * pr / ((1. + pr) * (1. + pr)) * dlnk;
         nsTmp = kpk * (-2.0 / (1. + pr) * fT * logfc * g * g / ln10 * a / (b * b * b) * (b + 0.14 * a) + fT / ((1. + pr) * (1 + pr)));
       }
+      lnkinf_factor = Ctbaf - flf * nsTmp;
       dCdrho = nsTmp * r.baseEff * invM;
       for (size_t i = 0; i < r.tb_idx.size(); ++i) dCdrho += nsTmp * (r.tb_eff[i] * y[r.tb_idx[i]]);
       nsTmp *= rho;
@@ -572,6 +579,7 @@ void prod_rates_sens_exact(const Mech &m, double T, double rho, double Mmix, con
       dCdT = 0.;
       for (int s = 0; s < ns - 1; ++s) dCdY[s] = 0.;
     }
+    if (qa) qa[ridx] = Rnet * ((r.type == RK_LINDEMANN || r.type == RK_TROE) ? lnkinf_factor : Ctbaf);
     const double q = Rnet * Ctbaf;
     const double dqdrho = dRdrho * Ctbaf + dCdrho * Rnet;
     const double dqdT = dRdT * Ctbaf + dCdT * Rnet;
@@ -1098,6 +1106,14 @@ int gfo_reactor_jac_isobaric(void *h, const double *state, double p, double Tin,
                              double *rhs, double *jac) {
   try { reactor_jac_isobaric(*static_cast<Mech *>(h), state, p, Tin, yin, tau, Tf, Ts, hc, eps, sov, heat, open != 0, rhs, jac); }
   catch (...) { return 2; }
+  return 0;
+}
+
+// d(rate of progress)/d(ln A) of every reaction (griffon order) at (rho, T, y)
+int gfo_rate_of_progress_log_a_derivative(void *h, double rho, double T, const double *y, double *qa) {
+  const Mech &m = *static_cast<Mech *>(h);
+  std::vector<double> w(m.ns), ws((m.ns + 1) * (m.ns + 1));
+  try { prod_rates_sens_exact(m, T, rho, mmw_of(m, y), y, w.data(), ws.data(), false, qa); } catch (...) { return 2; }
   return 0;
 }
 

@@ -68,6 +68,7 @@ class CpuGriffon:
     def __init__(self, cm, ref=False):
         self.cm = cm
         self.ns = cm.ns
+        self.nr = cm.nr
         self.ref = ref
         self.lib, self.p = _lib(ref)
         mw = np.ascontiguousarray(cm.mw, dtype=np.float64)
@@ -153,6 +154,15 @@ class CpuGriffon:
                                                                                       eps, sov, heat, open_),
                                                _dp(rhs), _dp(jac)) == 0
         return rhs, jac
+
+    def rate_of_progress_log_a_derivative(self, rho, T, y):
+        """d q_r / d ln A_r for every reaction, griffon order (the oracle restatement only)"""
+        assert not self.ref
+        qa = np.zeros(self.nr)
+        assert self._f('rate_of_progress_log_a_derivative')(self.h, ctypes.c_double(rho), ctypes.c_double(T),
+                                                             _dp(np.ascontiguousarray(y, dtype=np.float64)),
+                                                             _dp(qa)) == 0
+        return qa
 
     def prod_rates_sens_opt(self, rho, T, y, option):
         ws = np.zeros((self.ns + 1) ** 2)

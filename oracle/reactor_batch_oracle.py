@@ -4,8 +4,9 @@ integration and its discrete adjoint. The step loop and the adjoint recursion ar
 Newton-step kernels are restated here elementwise (x -= d where unconverged; the stage residual and its
 convergence test).
 
-The oracle has no per-reaction rate-of-progress output, so d(q_r)/d(ln A_r) is taken here from central
-differences of the oracle's production rates over mechanisms with A_r scaled by exp(+-eps):
+d(q_r)/d(ln A_r) comes from the oracle's exact per-reaction derivative (gfo_rate_of_progress_log_a_derivative),
+itself pinned against central differences of the oracle's production rates over mechanisms with A_r scaled by
+exp(+-eps) (tests/test_reactor_adjoint.py); qa='differences' uses those differences directly:
 d w_k / d ln A_r = M_k nu_kr dq_r/dln A_r for any species k in reaction r."""
 import copy
 
@@ -44,9 +45,10 @@ class _CpuLU:
 class OracleReactorBatch(ReactorBatch):
     _device = 'cpu'
 
-    def __init__(self, mech_spec, T0, Y0, pressure=101325., qa_eps=1.e-5):
+    def __init__(self, mech_spec, T0, Y0, pressure=101325., qa_eps=1.e-5, qa='exact'):
         self._cpu = CpuGriffon(CompiledMechanism(mech_spec.mech_data, order='name'))
         self._qa_eps = qa_eps
+        self._qa_mode = qa
         self._perturbed = None
         self._pressure = float(pressure)
         super().__init__(mech_spec, T0, Y0, pressure)
@@ -131,6 +133,9 @@ class OracleReactorBatch(ReactorBatch):
             qa.copy_(torch.from_numpy(self._qa(X, A[:, 0])))
 
     def _qa(self, X, rho):
+        if self._qa_mode == 'exact':
+            return np.stack([self._cpu.rate_of_progress_log_a_derivative(rho[r], X[r, 0], self._full_y(X[r]))
+                             for r in range(self.R)])
         if self._perturbed is None:
             md = self._mech.mech_data
             e = self._qa_eps

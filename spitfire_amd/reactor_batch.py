@@ -168,6 +168,7 @@ class ReactorBatch:
         recs = []
         nsteps = 0
         self.stats = {'steps': 0, 'newton_iterations': 0, 'max_newton_iterations_hit': 0}
+        active_its = torch.zeros((), dtype=torch.int64, device=self.dev)
         while True:
             if dt_schedule is not None:
                 if nsteps >= len(dt_schedule):
@@ -196,11 +197,13 @@ class ReactorBatch:
                 # reactors not yet converged, f(x), residual and convergence test (fused kernels)
                 self._stage_residual(h, g, fi, prior, xi, xn, r, conv, False, nonlinear_solve_tolerance)
                 for it in range(max_nonlinear_iter):
+                    # an iteration counts when some reactor is still unconverged at its start (device-side count,
+                    # no synchronization)
+                    active_its += (~conv).any()
                     self.lu.solve(r, d, st)
                     self._masked_update(xi, d, conv)
                     self._rhs(xi, fi)
                     self._stage_residual(h, g, fi, prior, xi, xn, r, conv, True, nonlinear_solve_tolerance)
-                    self.stats['newton_iterations'] += 1
                     # whether all reactors have converged is read back (a host synchronization) every other
                     # iteration: an iteration after all have converged changes nothing, every update being
                     # masked, so the results are those of checking every iteration
@@ -234,6 +237,7 @@ class ReactorBatch:
                 print('reactor batch step %d: min t %.3e' % (nsteps, float(t.min())), file=sys.stderr, flush=True)
         self.stats['steps'] = nsteps
         self.records = recs if record else None
+        self.stats['newton_iterations'] = int(active_its.item())
         self.final_state = x
         self.final_time = t
         return x

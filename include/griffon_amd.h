@@ -202,6 +202,15 @@ int gf_batch_reactor_general_noalloc(gf_mech *m, int n, int isochoric, const dou
                                      double eps_rad, double surface_area_over_volume, int heat_transfer_option,
                                      int open, int rates_sensitivity_option, double *rhs, double *jac, void *stream);
 
+/* Mixture properties of n states, device pointers, stream-ordered: Ty[n][ns+1] holds T then the ns mass fractions
+ * of each state; p[n] and rho[n] may each be NULL (then the density-from-pressure, resp. pressure-from-density,
+ * entry is computed from 0). out[n][8] = {mixture molecular weight, ideal-gas density from p, ideal-gas pressure from
+ * rho, cp, cv, enthalpy, energy, d(cp)/dT} per state, by the formulas of the mixture getters above
+ * (griffon.pyx:684-758). The batched form of those getters, for the property evaluators of
+ * src/spitfire/chemistry/analysis.py:77-220 (which go through Cantera's SolutionArray in the reference). */
+int gf_batch_mixture_properties(gf_mech *m, int n, const double *Ty, const double *p, const double *rho, double *out,
+                                void *stream);
+
 /* Batched block-tridiagonal (BTDDOD) factor/solve over nsys independent systems of possibly different
  * block counts: system s has nblocks[s] blocks of size bs; its values start at values + voff[s] (griffon
  * layout: nblocks*bs*bs block entries, then (nblocks-1)*bs sub, then (nblocks-1)*bs sup), its L blocks at

@@ -77,6 +77,7 @@ SIGNATURES = {
     'gf_batch_btddod_full_solve': [_int, _V, _int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V],
     'gf_batch_reactor_jac_isobaric_terms': [_V, _int, _V, _long, _long, _V, _long, _V, _long, _long, _V, _long, _V, _V, _V, _V, _V],
     'gf_batch_btddod_inverse_factorize': [_int, _V, _int, _V, _V, _V, _V, _int, _V],
+    'gf_batch_mixture_properties': [_V, _int, _V, _V, _V, _V, _V],
     'gf_batch_eigenvalue_bound': [_V, _int, _int, _V, _dbl, _V, _V, _V],
     'gf_batch_btddod_inverse_solve': [_int, _V, _int, _V, _V, _V, _V, _int, _V, _V, _V, _V],
     'gf_psitc_options_default': [_V],

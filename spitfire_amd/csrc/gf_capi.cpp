@@ -624,6 +624,15 @@ int gf_batch_reactor_general(gf_mech *m, int n, int isochoric, const double *x, 
                        static_cast<double *>(m->work), 0, static_cast<hipStream_t>(stream));
 }
 
+int gf_batch_mixture_properties(gf_mech *m, int n, const double *Ty, const double *p, const double *rho,
+                                double *out, void *stream) {
+  if (!m) return fail(GF_ERR_ARG, "null mechanism");
+  if (n < 0 || (n > 0 && (!Ty || !out))) return fail(GF_ERR_ARG, "bad mixture property arguments");
+  if (n == 0) return GF_OK;
+  GF_TRY(ensure_device(m));
+  return launch_mixture_properties(m, n, Ty, p, rho, out, static_cast<hipStream_t>(stream));
+}
+
 int gf_batch_btddod_full_factorize(int nsys, const int *nblocks, int bs, double *values, const long *voff,
                                    double *l_values, const long *loff, int *pivots, const long *poff, void *stream) {
   return launch_btd_factor(nsys, nblocks, bs, values, voff, l_values, loff, pivots, poff,

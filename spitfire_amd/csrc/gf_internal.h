@@ -187,6 +187,8 @@ int batch_general(const gf_mech *m, int n, int isochoric, const double *x, long 
 // griffon's thermodynamic getters for in = [T, y_0..y_{ns-1}] (device): out = {M, rho(p), p(rho), cp, cv, h, e,
 // dcp/dT, x[ns], cp[ns], cv[ns], h[ns], e[ns], dcp/dT[ns]}
 int launch_thermo_getters(const gf_mech *m, const double *in, double p, double rho, double *out, hipStream_t st);
+int launch_mixture_properties(const gf_mech *m, int n, const double *Ty, const double *p, const double *rho,
+                              double *out, hipStream_t st);
 int set_chem_launch_shape(int shape);  // -1: timed per launch kind and size class; 0..4: fixed
 int launch_chem(const gf_mech *m, const ChemIO &io, hipStream_t st);
 

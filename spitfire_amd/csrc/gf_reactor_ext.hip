@@ -193,15 +193,23 @@ __global__ void __launch_bounds__(64) reactor_ext_kernel(MechView mv, ReactorExt
 
 // one (T, y) per launch, one 64-thread workgroup: out = {mmw, density(p), pressure(rho), cp, cv, h, e, dcp/dT,
 // x[ns], cp[ns], cv[ns], h[ns], e[ns], dcp/dT[ns]}
-__global__ void __launch_bounds__(64) thermo_getters_kernel(MechView mv, const double *in, double p, double rho,
-                                                            double *out) {
+// One state per workgroup: state b reads in[b * in_stride] = {T, y_0 .. y_{ns-1}} and, when given, p[b] and rho[b]
+// (else the scalars p0 and rho0). Mixture values go to out[b * out_stride + 0..7]; species values to out + 8 .. 8 + 6 ns
+// when with_species is set.
+__global__ void __launch_bounds__(64) thermo_getters_kernel(MechView mv, const double *in, long in_stride,
+                                                            const double *pv, const double *rhov, double p0,
+                                                            double rho0, double *out, long out_stride,
+                                                            int with_species) {
   const int ns = mv.ns;
   const int t = threadIdx.x;
+  const long b = blockIdx.x;
+  in += b * in_stride;
+  out += b * out_stride;
   const double T = in[0];
   const double *y = in + 1;
-  double *x = out + 8, *cpo = x + ns, *cvo = cpo + ns, *ho = cvo + ns, *eo = ho + ns, *cso = eo + ns;
+  const double p = pv ? pv[b] : p0, rho = rhov ? rhov[b] : rho0;
   const double Ru = mv.Ru;
-  __shared__ double mm;
+  __shared__ double mm, scp[64], sh[64], se[64], scs[64];
   if (t == 0) {  // 1 / (sum y_i / M_i), combustion_kernels.h:381-387
     double d = 0.;
     for (int i = 0; i < ns; ++i) d += y[i] * mv.invmw[i];
@@ -211,23 +219,30 @@ __global__ void __launch_bounds__(64) thermo_getters_kernel(MechView mv, const d
   if (t < ns) {
     double c, cs, h, g, d;
     species_thermo(mv, t, T, log(T), 1. / T, c, cs, h, g, d);
-    x[t] = y[t] * mm * mv.invmw[t];
-    cpo[t] = c;
-    cvo[t] = c - Ru * mv.invmw[t];
-    ho[t] = h;
-    eo[t] = h - Ru * T * mv.invmw[t];
     // cp_sens_T: zero outside [Tmin, Tmax] and for constant cp (thermodynamics_kernels.cpp:183-260)
-    cso[t] = cs;
+    scp[t] = c;
+    sh[t] = h;
+    se[t] = h - Ru * T * mv.invmw[t];
+    scs[t] = cs;
+    if (with_species) {
+      double *x = out + 8, *cpo = x + ns, *cvo = cpo + ns, *ho = cvo + ns, *eo = ho + ns, *cso = eo + ns;
+      x[t] = y[t] * mm * mv.invmw[t];
+      cpo[t] = c;
+      cvo[t] = c - Ru * mv.invmw[t];
+      ho[t] = h;
+      eo[t] = se[t];
+      cso[t] = cs;
+    }
   }
   __syncthreads();
   if (t == 0) {
     double cp = 0., h = 0., e = 0., cs = 0.;
     for (int i = 0; i < ns; ++i) {
-      cp += y[i] * cpo[i];
-      h += ho[i] * y[i];
-      e += eo[i] * y[i];
+      cp += y[i] * scp[i];
+      h += sh[i] * y[i];
+      e += se[i] * y[i];
       // a constant-cp species resets the accumulated derivative (thermodynamics_kernels.cpp:200-203)
-      cs = mv.cpkind[i] == CP_CONST ? 0. : cs + y[i] * cso[i];
+      cs = mv.cpkind[i] == CP_CONST ? 0. : cs + y[i] * scs[i];
     }
     out[0] = mm;
     out[1] = p * mm / (T * Ru);
@@ -256,8 +271,18 @@ int launch_reactor_ext(const gf_mech *m, const ReactorExtIO &a, hipStream_t st) 
 }
 
 int launch_thermo_getters(const gf_mech *m, const double *in, double p, double rho, double *out, hipStream_t st) {
-  hipLaunchKernelGGL(thermo_getters_kernel, dim3(1), dim3(64), 0, st, m->view, in, p, rho, out);
+  hipLaunchKernelGGL(thermo_getters_kernel, dim3(1), dim3(64), 0, st, m->view, in, 0L, nullptr, nullptr, p, rho, out,
+                     0L, 1);
   if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, "thermodynamics kernel launch failed");
+  return GF_OK;
+}
+
+int launch_mixture_properties(const gf_mech *m, int n, const double *Ty, const double *p, const double *rho,
+                              double *out, hipStream_t st) {
+  if (n <= 0) return GF_OK;
+  hipLaunchKernelGGL(thermo_getters_kernel, dim3((unsigned)n), dim3(64), 0, st, m->view, Ty, (long)m->view.ns + 1,
+                     p, rho, 0., 0., out, 8L, 0);
+  if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, "mixture properties kernel launch failed");
   return GF_OK;
 }
 

@@ -56,7 +56,7 @@ class FlameletSpec:
                  max_dissipation_rate=None, stoich_dissipation_rate=None, dissipation_rate_form='Peters',
                  dissipation_rate=None, grid=None, grid_points=None, grid_type='clustered',
                  grid_cluster_intensity=4., grid_cluster_point='stoichiometric', heat_transfer='adiabatic',
-                 include_enthalpy_flux=False, include_variable_cp=False, rates_sensitivity_type='dense',
+                 include_enthalpy_flux=True, include_variable_cp=True, rates_sensitivity_type='dense',
                  sensitivity_transform_type='exact'):
         if heat_transfer != 'adiabatic':
             raise NotImplementedError('nonadiabatic flamelets are not supported by this implementation')
@@ -78,16 +78,6 @@ class FlameletSpec:
         self.include_variable_cp = include_variable_cp
         self.rates_sensitivity_type = rates_sensitivity_type
         self.sensitivity_transform_type = sensitivity_transform_type
-
-
-class FlameletHistory(dict):
-    """Fields of a transient flamelet run, each an array [time, mixture fraction] including the boundary
-    streams (the reference returns a spitfire Library with these dimensions, flamelet.py:1187-1211)."""
-
-    def __init__(self, t, z):
-        super().__init__()
-        self.time_values = np.asarray(t)
-        self.mixture_fraction_values = np.asarray(z)
 
 
 class Flamelet:
@@ -118,9 +108,17 @@ class Flamelet:
             self._z, self._dz = clustered_grid(fs.grid_points, zc, fs.grid_cluster_intensity)
         self._nz_interior = self._z.size - 2
         self._n_dof = self._n_equations * self._nz_interior
+        # dissipation rate (flamelet.py:427-468): given values, or a Peters/constant profile from the maximum or
+        # stoichiometric value, or zero when neither is given or the form is not one of those
         if fs.dissipation_rate is not None:
             self._x = np.array(fs.dissipation_rate, dtype=np.float64)
             self._max_dissipation_rate = np.max(self._x)
+            self._dissipation_rate_form = 'custom'
+        elif fs.dissipation_rate_form not in ('peters', 'Peters', 'constant') or \
+                (fs.max_dissipation_rate is None and fs.stoich_dissipation_rate is None):
+            self._x = np.zeros_like(self._z)
+            self._max_dissipation_rate = 0.
+            self._dissipation_rate_form = 'unspecified-set-to-0'
         else:
             if fs.max_dissipation_rate is not None:
                 self._max_dissipation_rate = fs.max_dissipation_rate
@@ -129,6 +127,7 @@ class Flamelet:
                 self._max_dissipation_rate = fs.stoich_dissipation_rate / np.exp(-2. * (erfinv(2. * zst - 1.)) ** 2)
             else:
                 self._max_dissipation_rate = fs.stoich_dissipation_rate
+            self._dissipation_rate_form = fs.dissipation_rate_form
             self._x = compute_dissipation_rate(self._z, self._max_dissipation_rate, fs.dissipation_rate_form)
         self._lewis_numbers = np.ones(self._n_species)
         self._include_enthalpy_flux = fs.include_enthalpy_flux
@@ -151,7 +150,9 @@ class Flamelet:
         self._block_thomas_l_values = np.zeros(n * n * nzi)
         self._block_thomas_d_pivots = np.zeros(self._jac_nelements_griffon, dtype=np.int32)
         self._iteration_count = None
+        self._initial_time = 0.
         self._current_time = 0.
+        self._solution_times = []
         self._jacobian_values = None
 
     def _setup_stencils(self):
@@ -299,7 +300,7 @@ class Flamelet:
             return None, None, False
         self._current_state = np.copy(state)
         self._iteration_count = iteration_count
-        return state, iteration_count, True
+        return self.make_library_from_interior_state(state), iteration_count, True
 
     def steady_solve_psitc(self, initial_guess=None, tolerance=1.e-6, max_iterations=400,
                            min_allowable_state_var=-1.e-6, ds_init=1., ds_init_decrease=4., adaptive_restart=True,
@@ -363,29 +364,62 @@ class Flamelet:
             return None, None, False, np.min(ds)
         self._iteration_count = iteration_count
         self._current_state = np.copy(state)
-        return state, iteration_count, True, np.min(ds)
+        return self.make_library_from_interior_state(state), iteration_count, True, np.min(ds)
 
     def compute_steady_state(self, tolerance=1.e-6, verbose=False, use_psitc=True, newton_args=None,
                              psitc_args=None, transient_args=None):
         """Newton, then pseudo-transient continuation, then ESDIRK64 time stepping to steady state
-        (flamelet.py:1705-1778); returns the converged interior state."""
+        (flamelet.py:1705-1778); returns a Library of temperature, pressure and mass fractions over the mixture
+        fraction grid (boundary streams included), and leaves the state in current_interior_state."""
         nargs = {'tolerance': tolerance, 'verbose': verbose, 'max_iterations': 38}
         nargs.update(newton_args or {})
-        state, it, conv = self.steady_solve_newton(**nargs)
+        lib, it, conv = self.steady_solve_newton(**nargs)
         if conv:
-            return state
+            return lib
         mds = 1.e-6
         if use_psitc:
             pargs = {'tolerance': tolerance, 'verbose': verbose, 'max_iterations': 400}
             pargs.update(psitc_args or {})
-            state, it, conv, mds = self.steady_solve_psitc(**pargs)
+            lib, it, conv, mds = self.steady_solve_psitc(**pargs)
             if conv:
-                return state
+                return lib
         targs = {'steady_tolerance': tolerance, 'transient_tolerance': 1.e-8, 'max_time_step': 1.e4,
                  'first_time_step': 1.e-2 * mds, 'maximum_steps_per_jacobian': 10, 'save_first_and_last_only': True}
         targs.update(transient_args or {})
         self.integrate_to_steady(**targs)
-        return self.current_interior_state
+        return self.make_library_from_interior_state(self.current_interior_state)
+
+    def make_library_from_interior_state(self, state_in):
+        """Library over the mixture fraction grid with temperature, pressure and every mass fraction, the
+        boundary streams at the ends (flamelet.py:1330-1350)."""
+        return self._fields_library(np.asarray(state_in)[None, :], None)
+
+    def _fields_library(self, states, t):
+        """temperature, pressure and mass fractions (the last species by difference) for interior states
+        [n, n_dof], over (time, mixture fraction) when t is given and over mixture fraction alone otherwise"""
+        from .library import Dimension, Library
+        zdim = Dimension('mixture_fraction', self._z)
+        lib = Library(Dimension('time', t), zdim) if t is not None else Library(zdim)
+        lib.extra_attributes['mech_spec'] = self._mechanism
+        ne = self._n_equations
+        names = self._mechanism.species_names
+
+        def field(interior, left, right):
+            a = np.empty((states.shape[0], self._z.size))
+            a[:, 0], a[:, -1] = left, right
+            a[:, 1:-1] = interior
+            return a if t is not None else a[0]
+
+        T = field(states[:, ::ne], self._oxy_stream.T, self._fuel_stream.T)
+        lib['temperature'] = T
+        lib['pressure'] = np.zeros_like(T) + self._pressure
+        last = np.ones_like(T)
+        for i, sp in enumerate(names[:-1]):
+            Y = field(states[:, 1 + i::ne], self._oxy_stream.Y[i], self._fuel_stream.Y[i])
+            lib['mass fraction ' + sp] = Y
+            last = last - Y
+        lib['mass fraction ' + names[-1]] = last
+        return lib
 
     # ---------------- transient integration (flamelet.py:1046-1237) ----------------
     def _setup_block_thomas(self, t, state_interior, prefactor):
@@ -402,8 +436,9 @@ class Flamelet:
                   return_info=False):
         """ESDIRK64 with a frozen-Jacobian Newton solver, PI step control and block-Thomas linear solves of
         dt*gamma*J - I (flamelet.py:1046-1213, linear_solver='block thomas'); negative entries are clipped after
-        every step. Starts from the current state and time and advances them. Returns a FlameletHistory with
-        the time values and, per field, an array [time, mixture fraction] that includes the boundary streams."""
+        every step. Starts from the current state and time and advances them. Returns a Library over (time,
+        mixture fraction) of temperature, pressure and mass fractions, boundary streams included
+        (flamelet.py:1187-1211)."""
         from .time_integration import KennedyCarpenterS6P4Q3, PIController, SimpleNewtonSolver, odesolve
         w = 1. / self._variable_scales
         newton = SimpleNewtonSolver(evaluate_jacobian_every_iter=False, norm_weighting=w,
@@ -428,23 +463,9 @@ class Flamelet:
             ts, states, info = out
         self._current_state = np.copy(states[-1])
         self._current_time = float(ts[-1])
+        self._solution_times = list(ts)
         self.info = info
-        hist = FlameletHistory(ts, self._z)
-        ne = self._n_equations
-        names = self._mechanism.species_names
-        T = np.empty((ts.size, self._z.size))
-        T[:, 0], T[:, -1] = self._oxy_stream.T, self._fuel_stream.T
-        T[:, 1:-1] = states[:, ::ne]
-        hist['temperature'] = T
-        hist['pressure'] = np.full_like(T, self._pressure)
-        last = np.ones_like(T)
-        for i, sp in enumerate(names[:-1]):
-            Y = np.empty_like(T)
-            Y[:, 0], Y[:, -1] = self._oxy_stream.Y[i], self._fuel_stream.Y[i]
-            Y[:, 1:-1] = states[:, 1 + i::ne]
-            hist['mass fraction ' + sp] = Y
-            last -= Y
-        hist['mass fraction ' + names[-1]] = last
+        hist = self._fields_library(states, ts)
         return (hist, info) if return_info else hist
 
     def integrate_to_steady(self, steady_tolerance=1.e-4, **kwargs):
@@ -452,6 +473,32 @@ class Flamelet:
 
     def integrate_to_time(self, final_time, **kwargs):
         return self.integrate(stop_at_time=final_time, **kwargs)
+
+    def _has_ignited(self, state, delta_temperature_ignition):
+        ne = self._n_equations
+        return np.max(state[::ne] - self._initial_state[::ne]) > delta_temperature_ignition
+
+    def integrate_to_steady_after_ignition(self, steady_tolerance=1.e-4, delta_temperature_ignition=400., **kwargs):
+        """integrate until steady after the temperature rose by delta_temperature_ignition somewhere
+        (flamelet.py:1239-1261)"""
+        def stop(t, state, residual, *args, **kw):
+            return self._has_ignited(state, delta_temperature_ignition) and residual < steady_tolerance
+
+        return self.integrate(stop_criteria=stop, **kwargs)
+
+    def compute_ignition_delay(self, delta_temperature_ignition=400., minimum_allowable_residual=1.e-12,
+                               return_solution=False, **kwargs):
+        """time at which the temperature first rises by delta_temperature_ignition; ValueError if the residual
+        drops below minimum_allowable_residual first (flamelet.py:1290-1328)"""
+        def stop(t, state, residual, *args, **kw):
+            if residual > minimum_allowable_residual:
+                return self._has_ignited(state, delta_temperature_ignition)
+            raise ValueError('From compute_ignition_delay(): residual < minimum allowable value (%g), suggesting that '
+                             'the flamelet will not ignite.' % minimum_allowable_residual)
+
+        lib = self.integrate(stop_criteria=stop, save_first_and_last_only=not return_solution, **kwargs)
+        tau = lib.time_values[-1]
+        return (tau, lib) if return_solution else tau
 
     # ---------------- accessors ----------------
     @property
@@ -472,6 +519,58 @@ class Flamelet:
     @property
     def dissipation_rate(self):
         return self._x.copy()
+
+    @property
+    def mechanism(self):
+        return self._mechanism
+
+    @property
+    def mixfrac_grid(self):
+        return self._z
+
+    def _state_2d_with_bcs(self, state):
+        """interior state with the oxidizer state prepended and the fuel state appended (flamelet.py:935-939)"""
+        return np.vstack((self._state_oxy, np.asarray(state).reshape((-1, self._n_equations)),
+                          self._state_fuel)).ravel()
+
+    def _mass_fraction_with_bcs(self, key, state):
+        """a mass fraction over the full grid from a full state; the last species by difference
+        (flamelet.py:923-933)"""
+        if isinstance(key, str):
+            key = self._mechanism.species_index(key)
+        if key == self._n_species - 1:
+            yn = np.ones(self._nz_interior + 2)
+            for i in range(1, self._n_equations):
+                yn -= state[i::self._n_equations]
+            return yn
+        return state[key + 1::self._n_equations]
+
+    @property
+    def initial_state(self):
+        return self._state_2d_with_bcs(self._initial_state)
+
+    @property
+    def initial_temperature(self):
+        return np.hstack((self._state_oxy[0], self._initial_state[::self._n_equations], self._state_fuel[0]))
+
+    def initial_mass_fraction(self, key):
+        return self._mass_fraction_with_bcs(key, self.initial_state)
+
+    @property
+    def current_state(self):
+        return self._state_2d_with_bcs(self._current_state)
+
+    def current_mass_fraction(self, key):
+        return self._mass_fraction_with_bcs(key, self.current_state)
+
+    @property
+    def solution_times(self):
+        """times of the last integrate() call (the reference keeps an always-empty list here, flamelet.py:615)"""
+        return np.array(self._solution_times)
+
+    def offset_time(self, delta):
+        self._initial_time += delta
+        self._current_time += delta
 
     @property
     def initial_interior_state(self):

@@ -353,6 +353,23 @@ class PyCombustionKernels:
         check(self._lib.gf_dcpdT_species(self._h, float(T), dptr(_f64(mass_fractions)), dptr(_f64(dcpdT_spec))))
 
     # ---------------- batched device entry points (torch tensors on the GPU) ----------------
+    def batch_mixture_properties(self, Ty, out, p=None, rho=None, stream=None):
+        """Ty: [n, ns+1] float64 CUDA tensor (T then the ns mass fractions); p, rho: None or [n]; out: [n, 8]
+        receives {molecular weight, density from p, pressure from rho, cp, cv, h, e, dcp/dT} per state."""
+        n = Ty.shape[0]
+        if Ty.shape != (n, self._ns + 1) or out.shape != (n, 8) or not Ty.is_contiguous() or not out.is_contiguous():
+            raise ValueError('Ty must be a contiguous [n, %d] tensor and out a contiguous [n, 8] tensor' % (self._ns + 1))
+        for a in (p, rho):
+            if a is not None and (a.shape != (n,) or not a.is_contiguous()):
+                raise ValueError('p and rho must be contiguous [n] tensors')
+        for a in (Ty, out, p, rho):
+            if a is not None and (not a.is_cuda or str(a.dtype) != 'torch.float64'):
+                raise ValueError('mixture property arrays must be float64 CUDA tensors')
+        check(self._lib.gf_batch_mixture_properties(self._h, n, Ty.data_ptr(),
+                                                    p.data_ptr() if p is not None else None,
+                                                    rho.data_ptr() if rho is not None else None, out.data_ptr(),
+                                                    _stream(stream)))
+
     def batch_reactor_rhs_isobaric(self, x, p, rhs, layout='soa', stream=None):
         """x, rhs: [ns, n] (layout 'soa') or [n, ns] ('aos') float64 CUDA tensors; p: [n] or [1]."""
         n, xs = _strides(x, self._ns, layout)

@@ -1,4 +1,4 @@
-"""A data-only reader for the reference's numpy gold pickles.
+"""A data-only reader for pickles of numpy data (the reference's gold vectors and Library files).
 
 It walks the pickle opcode stream itself (pickletools.genops parses, it does not execute) and builds plain
 Python values. It never resolves a GLOBAL to a Python object: the only globals it accepts are numpy's array,

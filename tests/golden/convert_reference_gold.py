@@ -21,7 +21,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
-from tests.golden.safe_pickle import Stub, load  # noqa: E402
+from spitfire_amd.safe_pickle import Stub, load  # noqa: E402
 
 
 def _plain(v):

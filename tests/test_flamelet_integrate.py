@@ -27,8 +27,10 @@ def _scaled(x, ns):
 def test_integrate_to_steady_lands_on_the_psitc_steady_state(mech_c1_data, oracle_lib, chi, npts):
     m, mk = _setup(mech_c1_data, chi, npts)
     f = mk()
-    st, it, ok, mds = f.steady_solve_psitc(tolerance=1e-9)
+    lib, it, ok, mds = f.steady_solve_psitc(tolerance=1e-9)
     assert ok
+    st = f.current_interior_state
+    assert np.array_equal(lib['temperature'][1:-1], st[::m.n_species])
     g = mk()
     hist, info = g.integrate_to_steady(steady_tolerance=1e-8, transient_tolerance=1e-8, max_time_step=1e4,
                                        first_time_step=1e-2 * mds, maximum_steps_per_jacobian=10, return_info=True)
@@ -46,10 +48,14 @@ def test_integrate_to_steady_lands_on_the_psitc_steady_state(mech_c1_data, oracl
 
 def test_compute_steady_state_falls_back_to_time_stepping(mech_c1_data, oracle_lib):
     m, mk = _setup(mech_c1_data, 1., 24)
-    ref, _, ok, _ = mk().steady_solve_psitc(tolerance=1e-9)
+    f = mk()
+    _, _, ok, _ = f.steady_solve_psitc(tolerance=1e-9)
     assert ok
+    ref = f.current_interior_state
     g = mk()
     # Newton given one iteration fails from the Burke-Schumann state; psitc is skipped, so time stepping decides
-    x = g.compute_steady_state(tolerance=1e-8, use_psitc=False, newton_args={'max_iterations': 1})
+    lib = g.compute_steady_state(tolerance=1e-8, use_psitc=False, newton_args={'max_iterations': 1})
+    x = g.current_interior_state
     assert g.info['steps'] > 0
+    assert lib.dim_names == ['mixture_fraction'] and np.array_equal(lib['temperature'][1:-1], x[::m.n_species])
     assert np.max(np.abs(x - ref) / _scaled(x, m.n_species)) <= 1e-8

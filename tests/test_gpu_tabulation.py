@@ -1,0 +1,145 @@
+"""Flamelet libraries and library analysis on the GPU.
+
+- build_adiabatic_slfm_library (tabulation.py:229-336) on the batched flamelet solver, in the reference's
+  sequential continuation and in the batched form, then the property evaluators (analysis.py:77-218): every field
+  of the reference's own adiabatic SLFM library (tests/tabulation/adiabatic_slfm) at the reference test's rtol 1e-6,
+  atol 1e-6, viscosity excepted (no transport here).
+- gf_batch_mixture_properties against the oracle's thermodynamic getters on seeded states of the 53-species
+  mechanism, in and out of the NASA7 ranges, given the pressure and given the density.
+- explosive_mode_analysis against eigenvalues of the oracle's reactor Jacobians.
+"""
+import numpy as np
+import pytest
+
+from tests.test_reference_gold import check_slfm_gold, slfm_setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _props(m, lib):
+    from spitfire_amd import analysis as sca
+    lib = sca.compute_specific_enthalpy(m, lib)
+    lib = sca.compute_isochoric_specific_heat(m, lib)
+    lib = sca.compute_isobaric_specific_heat(m, lib)
+    lib = sca.compute_density(m, lib)
+    return sca.compute_pressure(m, lib)
+
+
+@pytest.mark.parametrize('continuation, prec', [('sequential', 64), ('batched', 64), ('batched', 32)])
+def test_adiabatic_slfm_library_matches_reference_gold(continuation, prec):
+    from spitfire_amd.tabulation import build_adiabatic_slfm_library
+    print('adiabatic SLFM library, h2-burke, 34 points x 8 dissipation rates, continuation %s, fp%d'
+          % (continuation, prec), flush=True)
+    m, spec, chis = slfm_setup()
+    lib = build_adiabatic_slfm_library(spec, diss_rate_values=chis, verbose=False, continuation=continuation,
+                                       lu_precision=prec)
+    assert lib.extra_attributes['mech_spec'] is m and lib.dim('dissipation_rate_stoich').log_scaled
+    lib = _props(m, lib)
+    worst = check_slfm_gold(lib, m)
+    print('  largest |difference| / (1e-6 + 1e-6 |gold|): %.3e' % worst, flush=True)
+
+
+def test_slfm_library_stops_at_extinction():
+    """Dissipation rates past extinction: the table ends before the first extinguished flamelet, and
+    include_extinguished keeps it."""
+    from spitfire_amd.tabulation import build_adiabatic_slfm_library
+    m, spec, _ = slfm_setup()
+    chis = np.logspace(1, 5, 5)
+    print('SLFM past extinction, chi_st', chis, flush=True)
+    lib = build_adiabatic_slfm_library(spec, diss_rate_values=chis, verbose=False)
+    n = lib.dissipation_rate_stoich_npts
+    assert 1 <= n < chis.size
+    np.testing.assert_allclose(lib.dissipation_rate_stoich_values, chis[:n], rtol=1e-12)
+    T = lib['temperature']
+    Tlin = T[0, 0] + (T[-1, 0] - T[0, 0]) * lib.mixture_fraction_values
+    assert np.all(np.max(T - Tlin[:, None], axis=0) > 10.)
+    lib2 = build_adiabatic_slfm_library(spec, diss_rate_values=chis, verbose=False, include_extinguished=True)
+    assert lib2.dissipation_rate_stoich_npts == chis.size
+    print('  burning up to chi_st %.3g' % chis[n - 1], flush=True)
+
+
+def test_unreacted_and_burke_schumann_libraries():
+    from spitfire_amd.flamelet import Flamelet, FlameletSpec
+    from spitfire_amd.tabulation import build_adiabatic_bs_library, build_unreacted_library
+    m, spec, _ = slfm_setup()
+    for build, ic in ((build_unreacted_library, 'unreacted'), (build_adiabatic_bs_library, 'Burke-Schumann')):
+        lib = build(spec, verbose=False)
+        f = Flamelet(FlameletSpec(initial_condition=ic, **spec))
+        assert lib.dim_names == ['mixture_fraction'] and lib.mixture_fraction_npts == 34
+        np.testing.assert_array_equal(lib['temperature'], f.initial_temperature)
+        np.testing.assert_array_equal(lib['mass fraction H2O'], f.initial_mass_fraction('H2O'))
+
+
+def _random_states(m, n, seed):
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(250., 3600., n)  # includes temperatures outside the NASA7 ranges
+    Y = rng.dirichlet(np.full(m.n_species, 0.5), n)
+    p = 10. ** rng.uniform(4., 6.5, n)
+    return T, Y, p
+
+
+def test_batch_mixture_properties_vs_oracle(mech_53_data):
+    import torch
+    from oracle.oracle import CpuGriffon
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from spitfire_amd.mechdata import CompiledMechanism
+    m = ChemicalMechanismSpec(mech_53_data)
+    cpu = CpuGriffon(CompiledMechanism(m.mech_data, order='name'))
+    n = 2000
+    T, Y, p = _random_states(m, n, 7)
+    rho = np.random.default_rng(8).uniform(0.01, 10., n)
+    print('batched mixture properties, 53 species, %d states' % n, flush=True)
+    dev = torch.device('cuda')
+    Ty = torch.from_numpy(np.hstack((T[:, None], Y))).to(dev)
+    out = torch.empty((n, 8), dtype=torch.float64, device=dev)
+    m.griffon.batch_mixture_properties(Ty, out, p=torch.from_numpy(p).to(dev), rho=torch.from_numpy(rho).to(dev))
+    g = out.cpu().numpy()
+    ref = np.array([cpu.thermo_getters(T[i], Y[i], p[i], rho[i])[0] for i in range(n)])
+    err = np.abs(g - ref) / np.maximum(np.abs(ref), 1e-300)
+    print('  max relative difference per field', err.max(axis=0), flush=True)
+    assert err[:, :7].max() <= 1e-13
+    # dcp/dT is a sum of terms of both signs; compare against the sum of magnitudes
+    assert np.max(np.abs(g[:, 7] - ref[:, 7]) / np.maximum(np.abs(ref[:, 7]), 1.)) <= 1e-12
+    m.griffon.batch_mixture_properties(Ty, out)  # no pressure or density given: those entries come out zero
+    g2 = out.cpu().numpy()
+    assert np.all(g2[:, 1:3] == 0.) and np.array_equal(g2[:, [0, 3, 4, 5, 6, 7]], g[:, [0, 3, 4, 5, 6, 7]])
+
+
+@pytest.mark.parametrize('configuration', ['isobaric', 'isochoric'])
+def test_explosive_mode_analysis_vs_oracle(mech_53_data, configuration):
+    from scipy.linalg import eigvals
+    from oracle.oracle import CpuGriffon
+    from spitfire_amd import analysis as sca
+    from spitfire_amd.library import Dimension, Library
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from spitfire_amd.mechdata import CompiledMechanism
+    m = ChemicalMechanismSpec(mech_53_data)
+    cpu = CpuGriffon(CompiledMechanism(m.mech_data, order='name'))
+    n = 64
+    T, Y, p = _random_states(m, n, 11)
+    T = np.clip(T, 800., 2600.)
+    lib = Library(Dimension('state', np.arange(n, dtype=np.float64)))
+    lib['temperature'] = T
+    lib['pressure'] = p
+    mmw = 1. / (Y / m.molecular_weights[None, :]).sum(axis=1)
+    lib['density'] = p * mmw / (8314.462618 * T)
+    for i, s in enumerate(m.species_names):
+        lib['mass fraction ' + s] = Y[:, i]
+    print('CEMA %s, 53 species, %d states' % (configuration, n), flush=True)
+    sca.explosive_mode_analysis(m, lib, configuration=configuration, compute_explosion_indices=True,
+                                include_secondary_mode=True)
+    ns = m.n_species
+    for i in range(n):
+        if configuration == 'isobaric':
+            _, J = cpu.reactor_jac_isobaric(np.r_[T[i], Y[i, :-1]], p[i])
+            ne = ns
+        else:
+            _, J = cpu.reactor_jac_isochoric(np.r_[lib['density'][i], T[i], Y[i, :-1]])
+            ne = ns + 1
+        w = np.real(eigvals(J.reshape((ne, ne), order='F')))
+        w[(w > -1.e-4) & (w < 1.e-4)] = -np.inf
+        lam = np.max(w)
+        assert abs(lib["cema-lexp1"][i] - lam) <= 1e-6 * max(1., abs(lam)), (i, lib['cema-lexp1'][i], lam)
+    ei = np.array([lib['cema-ei1 T']] + [lib['cema-ei1 ' + s] for s in m.species_names[:-1]])
+    assert np.all(ei >= 0.) and np.all(ei.sum(axis=0) <= 1. + 1e-12)
+    assert 'cema-lexp2' in lib and 'cema-ei2 T' in lib

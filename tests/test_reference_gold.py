@@ -236,3 +236,78 @@ def test_closed_reactor_gold_on_oracle(key):
     r = OracleReactor(m, mix, configuration=conf, heat_transfer=heat, mass_transfer='closed')
     lib = r.integrate_to_steady()
     check_closed_reactor_gold(key, lib, m)
+
+
+def slfm_setup():
+    """The reference's adiabatic SLFM library test (tests/tabulation/adiabatic_slfm/rebless.py): h2-burke, air at
+    1200 K and H2 at 300 K, 1 atm, 34 clustered points, chi_st = logspace(0, 1, 8)."""
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    m = ChemicalMechanismSpec(cantera_input=os.path.join(GOLD, 'h2-burke.yaml'), group_name='h2-burke')
+    air = m.stream(stp_air=True)
+    air.TP = 1200., 101325.
+    fuel = m.stream('TPY', (300., 101325., 'H2:1'))
+    return m, {'mech_spec': m, 'oxy_stream': air, 'fuel_stream': fuel, 'grid_points': 34}, np.logspace(0, 1, 8)
+
+
+SLFM_GOLD_PROPS = ('temperature', 'pressure', 'enthalpy', 'heat capacity cv', 'heat capacity cp', 'density')
+
+
+def check_slfm_gold(lib, m, props=None):
+    """tests/tabulation/adiabatic_slfm/test.py: every property at rtol 1e-6, atol 1e-6 (viscosity excluded: no
+    transport here). Returns the largest |difference| / (atol + rtol |gold|) over the checked fields."""
+    gold = np.load(os.path.join(GOLD, 'ref_slfm_gold.npz'))
+    np.testing.assert_allclose(lib.mixture_fraction_values, gold['dim|mixture_fraction'], rtol=1e-13, atol=0.)
+    np.testing.assert_allclose(lib.dissipation_rate_stoich_values, gold['dim|dissipation_rate_stoich'], rtol=1e-13,
+                               atol=0.)
+    names = list(props if props is not None else SLFM_GOLD_PROPS) + ['mass fraction ' + s for s in m.species_names]
+    worst = 0.
+    for name in names:
+        g = gold['prop|' + name]
+        np.testing.assert_allclose(lib[name], g, rtol=1.e-6, atol=1.e-6, err_msg=name)
+        worst = max(worst, float(np.max(np.abs(lib[name] - g) / (1e-6 + 1e-6 * np.abs(g)))))
+    return worst
+
+
+def test_slfm_gold_on_oracle_continuation():
+    """The reference's SLFM continuation (tabulation.py:271-312: compute_steady_state at tolerance 1e-6 per
+    dissipation rate, each from its predecessor's state) run by the product's Flamelet on the oracle's flamelet
+    RHS/Jacobian with the reference's default model (enthalpy flux and variable cp on): temperature and mass
+    fractions of all eight flamelets against the reference's library. The first flamelet starts from Burke-Schumann
+    (the reference: Cantera equilibrium), so it lands within the solver tolerance of the gold; the rest follow the
+    reference's continuation step for step."""
+    from oracle.flamelet_oracle import OracleFlamelet
+    from spitfire_amd.flamelet import FlameletSpec
+    from spitfire_amd.library import Dimension, Library
+    m, spec, chis = slfm_setup()
+    ic = 'Burke-Schumann'
+    cols = []
+    for c in chis:
+        f = OracleFlamelet(FlameletSpec(stoich_dissipation_rate=c, initial_condition=ic, **spec))
+        cols.append(f.compute_steady_state(tolerance=1e-6))
+        ic = f.current_interior_state
+        assert np.max(f.current_temperature - f.linear_temperature) > 10.
+    lib = Library(Dimension('mixture_fraction', f.mixfrac_grid), Dimension('dissipation_rate_stoich', chis, True))
+    for q in cols[0].props:
+        lib[q] = np.stack([c[q] for c in cols], axis=1)
+    assert check_slfm_gold(lib, m, props=('temperature', 'pressure')) < 0.05
+
+
+def test_slfm_gold_thermo_on_oracle():
+    """Enthalpy, cp, cv and density of the reference's library (Cantera's SolutionArray at T, p, Y) against the
+    oracle's NASA7 getters at the gold's own T, p and Y."""
+    from oracle.oracle import CpuGriffon
+    from spitfire_amd.mechdata import CompiledMechanism
+    m, _, _ = slfm_setup()
+    gold = np.load(os.path.join(GOLD, 'ref_slfm_gold.npz'))
+    cpu = CpuGriffon(CompiledMechanism(m.mech_data, order='name'))
+    T, p = gold['prop|temperature'], gold['prop|pressure']
+    Y = np.stack([gold['prop|mass fraction ' + s] for s in m.species_names], axis=-1)
+    out = {k: np.empty_like(T) for k in ('enthalpy', 'heat capacity cp', 'heat capacity cv', 'density')}
+    for idx in np.ndindex(T.shape):
+        y = np.maximum(Y[idx], 0.)
+        y /= y.sum()
+        sc, _ = cpu.thermo_getters(T[idx], y, p[idx], 0.)
+        out['density'][idx], out['heat capacity cp'][idx], out['heat capacity cv'][idx], out['enthalpy'][idx] = \
+            sc[1], sc[3], sc[4], sc[5]
+    for k, v in out.items():
+        np.testing.assert_allclose(v, gold['prop|' + k], rtol=1e-10, atol=1e-6, err_msg=k)

@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
       rq[r] = Rnet * Ctbaf;
       if (a.qa) {
         const long q = a.sel ? (long)a.sel[i] : i;
-        a.qa[q * nr + r] = Rnet * ((R.type == RK_LINDEMANN || R.type == RK_TROE) ? qaC : Ctbaf);
+        a.qa[q * nr + R.orig] = Rnet * ((R.type == RK_LINDEMANN || R.type == RK_TROE) ? qaC : Ctbaf);
       }
       rq[nr + r] = dRdrho * Ctbaf + dCdrho * Rnet;
       rq[2 * nr + r] = dRdT * Ctbaf + dCdT * Rnet;

@@ -32,6 +32,7 @@ struct DevRxn {
   int n_rc, n_pd, n_net, n_tb, n_sp;
   int sumStoich, sumRc, sumPd;
   int tb_off, sp_off, slot_off, n_slot;
+  int orig;                       // the reaction's index in the mechanism (device order groups like reactions)
   int8_t tb_last;                 // index into the tb list of the last species, or -1
   int8_t pad[3];
   uint8_t rc_idx[NRC_MAX], rc_st[NRC_MAX];

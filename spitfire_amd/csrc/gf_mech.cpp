@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <set>
+#include <tuple>
 
 namespace gfa {
 
@@ -197,6 +198,29 @@ int finalize(gf_mech *m) {
   }
   if (nslot > 0xFFFE || nr > 0xFFFF || sgx.size() > 0xFFFFFFu)
     return fail(GF_ERR_UNSUPPORTED, "mechanism too large for the compact gather tables");
+  // Device order of the reactions: the rate kernels give a reaction (times a few states) to each lane, so
+  // reactions of one kind (special orders, type, Troe form, reversibility, rate form, reactant/product/third-body
+  // counts) are stored together and a wave takes one branch. Every table above was built in mechanism order
+  // and keeps that order (sums over reactions run as before); only the reaction records move, each keeping its
+  // mechanism index in `orig`, and the species gather is re-pointed at the new positions.
+  {
+    std::vector<int> perm(nr);
+    for (int r = 0; r < nr; ++r) perm[r] = r;
+    auto key = [&](int r) {
+      const DevRxn &d = drx[r];
+      return std::make_tuple(d.hasOrders, d.type, d.troeForm, d.reversible, d.kForm, d.n_rc, d.n_pd, d.n_tb);
+    };
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return key(a) < key(b); });
+    std::vector<int> pos(nr);
+    std::vector<DevRxn> sorted(nr);
+    for (int d = 0; d < nr; ++d) {
+      pos[perm[d]] = d;
+      sorted[d] = drx[perm[d]];
+      sorted[d].orig = perm[d];
+    }
+    drx.swap(sorted);
+    for (auto &e : sgx) e = (e & ~uint64_t(0xFFFF)) | (uint64_t)pos[e & 0xFFFFu];
+  }
   // byte image of the LDS tables
   std::vector<char> tbl;
   auto tput = [&](const void *src, size_t bytes) {

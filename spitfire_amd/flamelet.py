@@ -775,7 +775,8 @@ class FlameletBatch:
     def integrate_to_steady(self, steady_tolerance=1.e-4, lu_precision=64, select=None, first_time_step=1.e-6,
                             max_time_step=1.e-3, minimum_time_step_count=40, transient_tolerance=1.e-10,
                             maximum_steps_per_jacobian=10, nonlinear_solve_tolerance=1.e-12, max_nonlinear_iter=20,
-                            maximum_time_step_count=None, log_rate=0, raise_on_failure=True, **unused):
+                            maximum_time_step_count=None, stall_window=None, log_rate=0, raise_on_failure=True,
+                            **unused):
         """Flamelet.integrate_to_steady (flamelet.py:1215-1237; Flamelet.integrate with block-Thomas solves,
         flamelet.py:1046-1213; same defaults) for the selected flamelets, each from its initial guess, all on the
         GPU at once. Every flamelet keeps its own time, step size, stage Newton convergence, Jacobian-refresh
@@ -784,7 +785,15 @@ class FlameletBatch:
         block-Thomas solves of dt*gamma*J - I, PI step control (stepcontrol.py:50-117), negative entries clipped
         after every step, and a stop when |dx * 1/scale|_inf / dt < steady_tolerance. first_time_step: a scalar
         or one value per flamelet. Returns (states, converged, steps); converged where that test stopped the
-        flamelet (not the step limit)."""
+        flamelet (not the step limit).
+
+        Two optional bounds, neither in the reference (whose loop has no limit): maximum_time_step_count stops a
+        flamelet after that many steps, and stall_window stops one whose steadiness measure has not halved within
+        that many steps. They end the two ways a flamelet can fail to reach a steady tolerance that lies below the
+        rounding floor of its residual: the steadiness measure hovers at that floor, or the stage Newton solves
+        cannot reach nonlinear_solve_tolerance at the current step size, every step counts as slow (step size
+        x0.8) and the step size collapses. transient_stop[f] records why each flamelet stopped ('steady',
+        'step limit', 'stalled' or None if not selected)."""
         import ctypes
         import sys
         import torch
@@ -811,6 +820,9 @@ class FlameletBatch:
         running = sel.copy()
         steady = np.zeros(B, dtype=bool)
         resid = np.full(B, np.nan)
+        best = np.full(B, np.inf)        # steadiness at the last halving, and the step count then
+        best_step = np.zeros(B, dtype=np.int64)
+        stalled = np.zeros(B, dtype=bool)
         stats = {'steps': 0, 'nonlinear_iter': 0, 'projector_setups': 0, 'failed_steps': 0, 'sweeps': 0}
         mask = lambda m: np.ascontiguousarray(m, dtype=np.int32)
         on_dofs = lambda m: torch.from_numpy(np.asarray(m, dtype=bool)).to(dev)[seg]
@@ -903,7 +915,12 @@ class FlameletBatch:
             resid = np.where(run, res, resid)
             stop_max = run & (nsteps > maxsteps)
             steady = steady | (run & ~stop_max & (res < steady_tolerance))
-            running = running & ~stop_max
+            if stall_window is not None:
+                halved = acc & (res < 0.5 * best)
+                best = np.where(halved, res, best)
+                best_step = np.where(halved, nsteps, best_step)
+                stalled = stalled | (run & ~steady & ~stop_max & (nsteps - best_step > stall_window))
+            running = running & ~stop_max & ~stalled
             if log_rate and stats['sweeps'] % log_rate == 0:
                 left = running & ~steady
                 print('transient sweep %d: %d of %d flamelets stepping, max steps %d, min t %.3e, '
@@ -918,6 +935,8 @@ class FlameletBatch:
         self.residuals = np.where(sel, fnorm(fin), np.nan)
         self.transient_info = stats
         self.transient_times = t
+        self.transient_stop = np.array([('steady' if steady[f] else 'stalled' if stalled[f] else 'step limit')
+                                        if sel[f] else None for f in range(B)], dtype=object)
         conv = (sel & steady).astype(np.int32)
         return self._finish(sel.astype(np.int32), conv, nsteps.astype(np.int32), raise_on_failure,
                             'steady flamelet time stepping')
@@ -931,7 +950,10 @@ class FlameletBatch:
         initial guess with a first step of 1e-2 times psitc's final smallest pseudo-time step (skipped when
         use_time_stepping is False). Each stage is batched over its flamelets. Returns (states, converged,
         iterations);
-        method[f] names the stage that converged flamelet f ('newton', 'psitc', 'transient' or None)."""
+        method[f] names the stage that converged flamelet f ('newton', 'psitc', 'transient' or None).
+        Unlike the reference, the time-stepping stage is bounded by default: at most 5000 steps, and a flamelet
+        whose steadiness measure has not halved in 1000 steps is stopped as stalled (integrate_to_steady's
+        maximum_time_step_count and stall_window; transient_args can change or remove both)."""
         B = len(self._flamelets)
         self.residuals = np.full(B, np.nan)
         self.method = np.array([None] * B, dtype=object)
@@ -960,7 +982,8 @@ class FlameletBatch:
             self.stage_counters['psitc'] = self.counters
         if use_time_stepping and todo.any():
             targs = {'steady_tolerance': tolerance, 'transient_tolerance': 1.e-8, 'max_time_step': 1.e4,
-                     'first_time_step': 1.e-2 * mds, 'maximum_steps_per_jacobian': 10, 'log_rate': log_rate}
+                     'first_time_step': 1.e-2 * mds, 'maximum_steps_per_jacobian': 10, 'log_rate': log_rate,
+                     'maximum_time_step_count': 5000, 'stall_window': 1000}
             targs.update(transient_args or {})
             targs['raise_on_failure'] = False
             _, c3, st3 = self.integrate_to_steady(lu_precision=lu_precision, select=todo, **targs)

@@ -149,10 +149,15 @@ class ReactorBatch:
     # ---------------- forward ----------------
     def integrate(self, t_end, first_time_step=1.e-6, max_time_step=1.e6, transient_tolerance=1.e-10,
                   nonlinear_solve_tolerance=1.e-12, max_nonlinear_iter=20, dt_schedule=None, record=True,
-                  max_steps=100000, log_rate=0):
+                  max_steps=100000, log_rate=0, use_graphs=True):
         """Integrate every reactor from its initial state to t_end. With dt_schedule (a [steps, R] array of
         step sizes, 0 for no step) the step sizes are replayed instead of controlled (the frozen-step map that
-        the adjoint differentiates). Returns the final states [R, ns]."""
+        the adjoint differentiates). Returns the final states [R, ns].
+
+        With use_graphs, the stage Newton iterations run as a captured HIP graph of two iterations (block solve,
+        masked update, chemistry right-hand side, stage residual and convergence test), replayed until every
+        reactor has converged: one launch per two iterations instead of a dozen, over the same device buffers.
+        The results are bitwise those of the eager path (use_graphs=False)."""
         import torch
         R, ns = self.R, self.ns
         A, b, bh, g = self.A, self.b, self.bh, self.gamma
@@ -163,12 +168,35 @@ class ReactorBatch:
         f = torch.empty_like(x)
         J = torch.empty((R, ns * ns), dtype=torch.float64, device=self.dev)
         eye = torch.eye(ns, dtype=torch.float64, device=self.dev)
-        d = torch.empty_like(x)
         st = self._stream()
         recs = []
         nsteps = 0
         self.stats = {'steps': 0, 'newton_iterations': 0, 'max_newton_iterations_hit': 0}
         active_its = torch.zeros((), dtype=torch.int64, device=self.dev)
+        # the Newton iteration's operands live in fixed buffers (a captured graph reads fixed addresses)
+        M = torch.empty((R, ns * ns), dtype=torch.float64, device=self.dev)
+        hb = torch.empty(R, dtype=torch.float64, device=self.dev)
+        xn = torch.empty_like(x)
+        xi = torch.empty_like(x)
+        fi = torch.empty_like(x)
+        prior = torch.empty_like(x)
+        r = torch.empty_like(x)
+        d = torch.empty_like(x)
+        conv = torch.zeros(R, dtype=torch.bool, device=self.dev)
+        tol = nonlinear_solve_tolerance
+
+        its = torch.zeros(R, dtype=torch.int64, device=self.dev)  # iterations each reactor started unconverged
+
+        def newton_iteration():
+            # elementwise only (no reductions, which can record memory fills into a graph)
+            its.add_(~conv)
+            self.lu.solve(r, d, self._stream())
+            self._masked_update(xi, d, conv)
+            self._rhs(xi, fi)
+            self._stage_residual(hb, g, fi, prior, xi, xn, r, conv, True, tol)
+
+        use_graphs = use_graphs and self.dev.type == 'cuda'
+        pair = None  # the captured graph of two Newton iterations, made after the first eager pair
         while True:
             if dt_schedule is not None:
                 if nsteps >= len(dt_schedule):
@@ -180,40 +208,50 @@ class ReactorBatch:
                     break
                 dt = torch.where(t + dt > t_end, t_end - t, dt)
                 h = torch.where(active, dt, torch.zeros_like(dt))
-            xn = x.clone()
+            hb.copy_(h)
+            xn.copy_(x)
             self._jac(xn, f, J)
-            M = (h[:, None, None] * g * J.view(R, ns, ns) - eye).reshape(R, ns * ns).contiguous()
+            torch.sub(hb[:, None, None] * g * J.view(R, ns, ns), eye, out=M.view(R, ns, ns))
             self.lu.factor(M, st)
             ks = [f.clone()]
             X = [xn.clone()]
             xs = xn.clone()
             for s in range(1, 6):
-                prior = sum(A[s, j] * ks[j] for j in range(s)).contiguous()
-                xi = xs.clone()
-                fi = ks[-1].clone()
-                r = torch.empty_like(xi)
-                conv = torch.zeros(R, dtype=torch.bool, device=self.dev)
+                prior.copy_(sum(A[s, j] * ks[j] for j in range(s)))
+                xi.copy_(xs)
+                fi.copy_(ks[-1])
+                conv.zero_()
                 # stage residual r = h (gamma f + prior) - (x - xn), then Newton: x -= solve(r) for the
                 # reactors not yet converged, f(x), residual and convergence test (fused kernels)
-                self._stage_residual(h, g, fi, prior, xi, xn, r, conv, False, nonlinear_solve_tolerance)
-                for it in range(max_nonlinear_iter):
-                    # an iteration counts when some reactor is still unconverged at its start (device-side count,
-                    # no synchronization)
-                    active_its += (~conv).any()
-                    self.lu.solve(r, d, st)
-                    self._masked_update(xi, d, conv)
-                    self._rhs(xi, fi)
-                    self._stage_residual(h, g, fi, prior, xi, xn, r, conv, True, nonlinear_solve_tolerance)
-                    # whether all reactors have converged is read back (a host synchronization) every other
-                    # iteration: an iteration after all have converged changes nothing, every update being
-                    # masked, so the results are those of checking every iteration
-                    if (it % 2 == 1 or it == max_nonlinear_iter - 1) and bool(conv.all()):
+                self._stage_residual(hb, g, fi, prior, xi, xn, r, conv, False, tol)
+                # whether all reactors have converged is read back (a host synchronization) every other
+                # iteration: an iteration after all have converged changes nothing, every update being
+                # masked, so the results are those of checking every iteration
+                it = 0
+                while it < max_nonlinear_iter:
+                    if max_nonlinear_iter - it >= 2:
+                        if pair is not None:
+                            pair.replay()
+                        else:
+                            newton_iteration()
+                            newton_iteration()
+                            if use_graphs:
+                                pair = self._capture(lambda: (newton_iteration(), newton_iteration()))
+                        it += 2
+                    else:
+                        newton_iteration()
+                        it += 1
+                    if bool(conv.all()):
                         break
                 else:
                     self.stats['max_newton_iterations_hit'] += 1
+                # an iteration counts when some reactor was still unconverged at its start; a converged reactor
+                # stays converged, so that count is the largest per-reactor count (device-side, no synchronization)
+                active_its.add_(its.max())
+                its.zero_()
                 ks.append(fi.clone())
                 X.append(xi.clone())
-                xs = xi
+                xs = xi.clone()
             dx = h[:, None] * sum(b[j] * ks[j] for j in range(6))
             err = torch.max(torch.abs(h[:, None] * sum((b[j] - bh[j]) * ks[j] for j in range(6)) * w), dim=1).values
             ok = torch.isfinite(dx).all(dim=1)
@@ -241,6 +279,19 @@ class ReactorBatch:
         self.final_state = x
         self.final_time = t
         return x
+
+    @staticmethod
+    def _capture(fn):
+        """fn's device work captured as a graph on a side stream (it is not run by the capture)"""
+        import torch
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(graph, stream=side):
+                fn()
+        torch.cuda.current_stream().wait_stream(side)
+        return graph
 
     def step_sizes(self):
         """[steps, R] accepted step sizes of the last recorded integration (0: no step)"""

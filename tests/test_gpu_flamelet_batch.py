@@ -237,3 +237,25 @@ def test_time_stepping_stage_fp32_and_state_errors(mech_c1_data, oracle_lib):
     _native.check(lib.gf_flamelet_batch_transient_solve(fb._h, _native.iptr(flags), r.data_ptr(), o.data_ptr()))
     torch.cuda.synchronize()
     assert bool(torch.isfinite(o).all())
+
+
+def test_time_stepping_stage_bounds(mech_c1_data):
+    """The time-stepping stage's bounds, forced from the host with a steady tolerance no state can meet (1e-30):
+    a stall window stops each flamelet once its steadiness measure stops halving, a step cap stops it at that
+    many steps, and both report the flamelets as not converged with the reason in transient_stop."""
+    from spitfire_amd.flamelet import FlameletBatch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    m = ChemicalMechanismSpec(mech_c1_data)
+    chis, npts = [3., 30.], [24, 28]
+    say('time stepping bounds: stall window 25, then a 60-step cap')
+    fb = FlameletBatch(_flamelets(m, chis, npts))
+    _, conv, steps = fb.integrate_to_steady(steady_tolerance=1e-30, first_time_step=1e-8, transient_tolerance=1e-8,
+                                            max_time_step=1e4, maximum_time_step_count=100000, stall_window=25,
+                                            raise_on_failure=False)
+    say('  stall: steps', steps.tolist(), 'stop', list(fb.transient_stop))
+    assert not conv.any() and list(fb.transient_stop) == ['stalled', 'stalled'] and steps.max() < 100000
+    fb = FlameletBatch(_flamelets(m, chis, npts))
+    _, conv, steps = fb.integrate_to_steady(steady_tolerance=1e-30, first_time_step=1e-8, transient_tolerance=1e-8,
+                                            max_time_step=1e4, maximum_time_step_count=60, raise_on_failure=False)
+    say('  cap: steps', steps.tolist(), 'stop', list(fb.transient_stop))
+    assert not conv.any() and list(fb.transient_stop) == ['step limit', 'step limit'] and steps.tolist() == [61, 61]

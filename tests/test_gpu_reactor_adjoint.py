@@ -168,3 +168,28 @@ def test_heat_release_autograd_function(mech_c1_data):
             L0 = L.detach().clone()
         else:
             assert float((L.detach() - L0).abs().max()) > 1e-6 * float(L0.abs().max())
+
+
+def test_graph_captured_newton_iterations_match_eager(mech_c1_data):
+    """The stage Newton iterations replayed from a captured HIP graph (two iterations per replay, eager work and
+    host synchronizations between replays) give bitwise the eager path's states, step records and counts, for
+    controlled steps and for a replayed step schedule."""
+    import torch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from spitfire_amd.reactor_batch import ReactorBatch
+    m = ChemicalMechanismSpec(mech_c1_data)
+    rng = np.random.default_rng(5)
+    T0, Y0 = c1_reactors(m, rng.uniform(1000., 1400., 64), rng.uniform(0.5, 2., 64))
+    out = {}
+    for graphs in (False, True):
+        say('64 C1 reactors to 1e-4 s, graphs', graphs)
+        g = ReactorBatch(m, T0, Y0)
+        x = g.integrate(1.e-4, use_graphs=graphs)
+        sched = g.step_sizes()
+        g2 = ReactorBatch(m, T0, Y0)
+        x2 = g2.integrate(1.e-4, dt_schedule=sched, use_graphs=graphs)
+        out[graphs] = (x.cpu(), [tuple(a.cpu() for a in rec) for rec in g.records], dict(g.stats), x2.cpu())
+        say('  steps', g.stats)
+    (a, ra, sa, a2), (b, rb, sb, b2) = out[False], out[True]
+    assert torch.equal(a, b) and torch.equal(a2, b2) and sa == sb and len(ra) == len(rb)
+    assert all(torch.equal(p, q) for u, v in zip(ra, rb) for p, q in zip(u, v))

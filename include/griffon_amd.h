@@ -278,6 +278,12 @@ int gf_flamelet_batch_create(gf_mech *m, int nfl, const int *nzi, double pressur
                              const double *chi, const double *oxy, const double *fuel, int include_enthalpy_flux,
                              int include_variable_cp, gf_flamelet_batch **out);
 int gf_flamelet_batch_destroy(gf_flamelet_batch *b);
+/* Heat loss of nonadiabatic flamelets (the T_convection, h_convection, T_radiation, h_radiation and
+ * use_scaled_heat_loss arguments of flamelet_rhs / flamelet_jacobian, griffon.pyx:580-679, for every flamelet of
+ * the batch): host arrays of sum(nzi) values each, flamelet after flamelet over its interior points. Four nulls
+ * make the batch adiabatic again (its initial setting). */
+int gf_flamelet_batch_set_heat_loss(gf_flamelet_batch *b, const double *T_conv, const double *h_conv,
+                                    const double *T_rad, const double *h_rad, int use_scaled_heat_loss);
 /* interior states, point-major per flamelet, concatenated: sum(nzi)*ns doubles; set_state also sets the
  * initial guess the restarts return to */
 int gf_flamelet_batch_set_state(gf_flamelet_batch *b, const double *interior);

@@ -21,21 +21,29 @@ class OracleFlamelet(Flamelet):
         (self._maj_coeff_griffon, self._sub_coeff_griffon, self._sup_coeff_griffon, self._mcoeff_griffon,
          self._ncoeff_griffon) = st
 
+    def _oracle_heat_loss(self):
+        """(T_conv, h_conv, T_rad, h_rad), the oracle's (C++) order, or None when adiabatic"""
+        if self._heat_transfer == 'adiabatic':
+            return None
+        return self._T_conv, self._h_conv, self._T_rad, self._h_rad
+
     def _stencils(self):
         return (self._maj_coeff_griffon, self._sub_coeff_griffon, self._sup_coeff_griffon, self._mcoeff_griffon,
                 self._ncoeff_griffon)
 
-    def _adiabatic_rhs(self, t, state_interior):
+    def _flamelet_rhs(self, t, state_interior):
         return self._cpu.flamelet_rhs(np.ascontiguousarray(state_interior), self._pressure, self._state_oxy,
                                       self._state_fuel, self._nz_interior, self._stencils(), self._x,
-                                      incl_ef=self._include_enthalpy_flux, incl_vcp=self._include_variable_cp)
+                                      incl_ef=self._include_enthalpy_flux, incl_vcp=self._include_variable_cp,
+                                      heat_loss=self._oracle_heat_loss(), scaled_hl=self._scale_heat_loss_by_temp_range)
 
-    def _adiabatic_jac(self, state_interior, scale_and_offset=False, prefactor=0.):
+    def _flamelet_jac(self, state_interior, scale_and_offset=False, prefactor=0.):
         return self._cpu.flamelet_jacobian(np.ascontiguousarray(state_interior), self._pressure, self._state_oxy,
                                            self._state_fuel, self._nz_interior, self._stencils(), self._x,
                                            scale_and_offset=scale_and_offset, prefactor=prefactor,
                                            incl_ef=self._include_enthalpy_flux,
-                                           incl_vcp=self._include_variable_cp)[0]
+                                           incl_vcp=self._include_variable_cp, heat_loss=self._oracle_heat_loss(),
+                                           scaled_hl=self._scale_heat_loss_by_temp_range)[0]
 
     def _factorize(self, values):
         d, l, piv = self._cpu.btddod_full_factorize(values, self._nz_interior, self._n_equations)
@@ -58,4 +66,5 @@ class OracleFlamelet(Flamelet):
                                            self._state_fuel, self._nz_interior, self._stencils(), self._x,
                                            compute_eig=True, diffterm=diffterm,
                                            incl_ef=self._include_enthalpy_flux,
-                                           incl_vcp=self._include_variable_cp)
+                                           incl_vcp=self._include_variable_cp, heat_loss=self._oracle_heat_loss(),
+                                           scaled_hl=self._scale_heat_loss_by_temp_range)

@@ -61,6 +61,7 @@ struct gf_flamelet_batch {
          *cmaj = nullptr, *csub = nullptr, *csup = nullptr, *mco = nullptr, *nco = nullptr, *chi = nullptr,
          *dz = nullptr, *dp = nullptr, *fval = nullptr, *fres = nullptr, *fmin = nullptr, *fdiff = nullptr;
   float *D32 = nullptr;  // FP32 inverse diagonal blocks (mixed-precision path); L holds the FP64 ones
+  double *hl = nullptr;  // heat-loss parameters [4][NI] (own allocation), null while adiabatic
   int *piv = nullptr, *dnzi = nullptr, *flag = nullptr, *gmin = nullptr, *sel = nullptr, *sys_nb = nullptr;
   long *dpt0 = nullptr, *ddof0 = nullptr, *dipt0 = nullptr, *dchi0 = nullptr, *dvoff = nullptr, *ddz0 = nullptr,
        *js_off = nullptr, *eb = nullptr, *eo = nullptr, *sys_v = nullptr, *sys_l = nullptr, *sys_p = nullptr,
@@ -589,6 +590,7 @@ int gf_flamelet_batch_create(gf_mech *m, int nfl, const int *nzi, double pressur
   v.chem_rhs = b->chem_rhs;
   v.aux = b->aux;
   v.cpi = b->cpi;
+  v.NI = b->NI;
   v.NV = V;
   v.NJ = Jv;
   for (int f = 0; f < nfl; ++f) v.maxNJ = std::max(v.maxNJ, (long)ns * ((long)nzi[f] * ns + 2 * (nzi[f] - 1)));
@@ -607,8 +609,33 @@ int gf_flamelet_batch_create(gf_mech *m, int nfl, const int *nzi, double pressur
 int gf_flamelet_batch_destroy(gf_flamelet_batch *b) {
   if (!b) return GF_OK;
   if (b->dmem) hipFree(b->dmem);
+  if (b->hl) hipFree(b->hl);
   if (b->hpin) hipHostFree(b->hpin);
   delete b;
+  return GF_OK;
+}
+
+int gf_flamelet_batch_set_heat_loss(gf_flamelet_batch *b, const double *T_conv, const double *h_conv,
+                                    const double *T_rad, const double *h_rad, int use_scaled_heat_loss) {
+  if (!b) return fail(GF_ERR_ARG, "null flamelet batch");
+  const hipStream_t st = b->m->stream;
+  if (!T_conv && !h_conv && !T_rad && !h_rad) {
+    b->view.adiabatic = 1;
+    b->view.hl = nullptr;
+    return GF_OK;
+  }
+  if (!T_conv || !h_conv || !T_rad || !h_rad) return fail(GF_ERR_ARG, "all four heat-loss arrays are needed");
+  const size_t n = (size_t)b->NI;
+  if (!b->hl && hipMalloc(&b->hl, 4 * n * sizeof(double)) != hipSuccess)
+    return fail(GF_ERR_HIP, "hipMalloc failed for the heat-loss parameters");
+  const double *src[4] = {T_conv, h_conv, T_rad, h_rad};
+  for (int k = 0; k < 4; ++k)
+    if (hipMemcpyAsync(b->hl + k * n, src[k], n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+      return fail(GF_ERR_HIP, "host-to-device copy failed");
+  if (hipStreamSynchronize(st) != hipSuccess) return fail(GF_ERR_HIP, "stream synchronize failed");
+  b->view.hl = b->hl;
+  b->view.adiabatic = 0;
+  b->view.scaled_hl = use_scaled_heat_loss ? 1 : 0;
   return GF_OK;
 }
 

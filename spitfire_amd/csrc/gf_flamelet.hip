@@ -234,7 +234,14 @@ __device__ FlameletIO fb_io(const FlameletBatchView &b, int f, const double *X) 
   io.mco = b.mco + i0;
   io.nco = b.nco + i0;
   io.chi = b.chi + b.chi0[f];
-  io.adiabatic = 1;
+  io.adiabatic = b.adiabatic;
+  if (!b.adiabatic) {
+    io.Tc = b.hl + i0;
+    io.hc = b.hl + b.NI + i0;
+    io.Tr = b.hl + 2 * b.NI + i0;
+    io.hr = b.hl + 3 * b.NI + i0;
+    io.scaled_hl = b.scaled_hl;
+  }
   io.incl_ef = b.incl_ef;
   io.incl_vcp = b.incl_vcp;
   io.chem_rhs = b.chem_rhs + p0 * ns;

@@ -228,6 +228,10 @@ struct FlameletBatchView {
   const long *pt0 = nullptr, *dof0 = nullptr, *ipt0 = nullptr, *chi0 = nullptr, *voff = nullptr;
   const double *cmaj = nullptr, *csub = nullptr, *csup = nullptr, *mco = nullptr, *nco = nullptr, *chi = nullptr;
   const double *chem_rhs = nullptr, *aux = nullptr, *cpi = nullptr;
+  // heat loss (nonadiabatic flamelets): hl[k * NI + ipt0[f] + i] for k = T_conv, h_conv, T_rad, h_rad
+  int adiabatic = 1, scaled_hl = 0;
+  const double *hl = nullptr;
+  long NI = 0;          // total interior points
   long NV = 0, NJ = 0;  // total unknowns, total Jacobian entries
   long maxNJ = 0;       // Jacobian entries of the largest flamelet
 };

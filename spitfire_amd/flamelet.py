@@ -50,16 +50,28 @@ def clustered_grid(grid_points, grid_cluster_point, grid_cluster_intensity=6.):
 
 
 class FlameletSpec:
-    """Flamelet specification (flamelet.py:35-237), adiabatic flamelets."""
+    """Flamelet specification (flamelet.py:35-237). Nonadiabatic flamelets lose heat by convection,
+    h_conv (T - T_conv), and radiation, eps sigma (T^4 - T_rad^4), per interior point (flamelet_kernels.cpp:1121-1138,
+    1290-1320)."""
 
     def __init__(self, mech_spec=None, initial_condition='Burke-Schumann', oxy_stream=None, fuel_stream=None,
                  max_dissipation_rate=None, stoich_dissipation_rate=None, dissipation_rate_form='Peters',
                  dissipation_rate=None, grid=None, grid_points=None, grid_type='clustered',
                  grid_cluster_intensity=4., grid_cluster_point='stoichiometric', heat_transfer='adiabatic',
-                 include_enthalpy_flux=True, include_variable_cp=True, rates_sensitivity_type='dense',
-                 sensitivity_transform_type='exact'):
-        if heat_transfer != 'adiabatic':
-            raise NotImplementedError('nonadiabatic flamelets are not supported by this implementation')
+                 convection_temperature=None, radiation_temperature=None, convection_coefficient=None,
+                 radiative_emissivity=None, scale_heat_loss_by_temp_range=False, scale_convection_by_dissipation=False,
+                 use_linear_ref_temp_profile=False, include_enthalpy_flux=True, include_variable_cp=True,
+                 rates_sensitivity_type='dense', sensitivity_transform_type='exact'):
+        if heat_transfer not in ('adiabatic', 'nonadiabatic'):
+            raise ValueError('Flamelet specifications: Bad heat_transfer argument detected: ' + str(heat_transfer) +
+                             "; acceptable values: ['adiabatic', 'nonadiabatic']")
+        self.convection_temperature = convection_temperature
+        self.radiation_temperature = radiation_temperature
+        self.convection_coefficient = convection_coefficient
+        self.radiative_emissivity = radiative_emissivity
+        self.scale_heat_loss_by_temp_range = scale_heat_loss_by_temp_range
+        self.scale_convection_by_dissipation = scale_convection_by_dissipation
+        self.use_linear_ref_temp_profile = use_linear_ref_temp_profile
         self.mech_spec = mech_spec
         self.initial_condition = initial_condition
         self.oxy_stream = oxy_stream
@@ -130,6 +142,7 @@ class Flamelet:
             self._dissipation_rate_form = fs.dissipation_rate_form
             self._x = compute_dissipation_rate(self._z, self._max_dissipation_rate, fs.dissipation_rate_form)
         self._lewis_numbers = np.ones(self._n_species)
+        self._setup_heat_transfer(fs)
         self._include_enthalpy_flux = fs.include_enthalpy_flux
         self._include_variable_cp = fs.include_variable_cp
         self._rsopt = {'dense': 0, 'no-TBAF': 1, 'sparse': 2}[fs.rates_sensitivity_type]
@@ -154,6 +167,52 @@ class Flamelet:
         self._current_time = 0.
         self._solution_times = []
         self._jacobian_values = None
+
+    def _heat_transfer_array(self, value, name):
+        """a float -> that value at every interior point; an array -> a copy (flamelet.py:354-364)"""
+        if value is None:
+            raise ValueError('Flamelet specifications: Nonadiabatic heat transfer was selected but no ' + name +
+                             ' argument was given.')
+        if isinstance(value, float):
+            return value + np.zeros(self._nz_interior)
+        if isinstance(value, np.ndarray):
+            return np.array(value, dtype=np.float64)
+        raise ValueError(name + ' was not given as a float (constant) or numpy array')
+
+    def _setup_heat_transfer(self, fs):
+        """convection and radiation parameters at the interior points (flamelet.py:470-519)"""
+        self._heat_transfer = fs.heat_transfer
+        self._scale_heat_loss_by_temp_range = bool(getattr(fs, 'scale_heat_loss_by_temp_range', False))
+        self._T_conv = self._T_rad = self._h_conv = self._h_rad = None
+        if self._heat_transfer == 'adiabatic':
+            return
+        self._h_conv = self._heat_transfer_array(fs.convection_coefficient, 'convection_coefficient')
+        self._h_rad = self._heat_transfer_array(fs.radiative_emissivity, 'radiative_emissivity')
+        if fs.use_linear_ref_temp_profile:
+            self._T_conv = self._oxy_stream.T + self._z[1:-1] * (self._fuel_stream.T - self._oxy_stream.T)
+            self._T_rad = self._T_conv.copy()
+        else:
+            self._T_rad = self._heat_transfer_array(fs.radiation_temperature, 'radiation_temperature')
+            self._T_conv = self._heat_transfer_array(fs.convection_temperature, 'convection_temperature')
+        if fs.scale_convection_by_dissipation:
+            zst = self._mechanism.stoich_mixture_fraction(self._fuel_stream, self._oxy_stream)
+            factor = np.max(self._x) / (1. - zst) / zst
+            self._h_conv = self._h_conv * factor
+            self._h_rad = self._h_rad * factor
+        for a in (self._T_conv, self._T_rad, self._h_conv, self._h_rad):
+            if a.size != self._nz_interior:
+                raise ValueError('heat transfer parameters must have one value per interior grid point')
+
+    def _heat_loss_args(self):
+        """(adiabatic, T_convection, T_radiation, h_convection, h_radiation) in the griffon argument order"""
+        if self._heat_transfer == 'adiabatic':
+            null = np.zeros(1)
+            return True, null, null, null, null
+        return False, self._T_conv, self._T_rad, self._h_conv, self._h_rad
+
+    @property
+    def heat_transfer(self):
+        return self._heat_transfer
 
     def _setup_stencils(self):
         self._griffon.flamelet_stencils(self._dz, self._nz_interior, self._x, 1. / self._lewis_numbers,
@@ -200,44 +259,45 @@ class Flamelet:
         return st.ravel().copy()
 
     # ---------------- griffon wrappers (flamelet.py:642-762, 911-915) ----------------
-    def _adiabatic_rhs(self, t, state_interior):
+    # the reference's _adiabatic_* and _nonadiabatic_* wrappers in one: the heat-loss arguments decide
+    def _flamelet_rhs(self, t, state_interior):
         rhs = np.zeros(self._n_dof)
-        null = np.zeros(1)
-        self._griffon.flamelet_rhs(state_interior, self._pressure, self._state_oxy, self._state_fuel, True, null, null,
-                                   null, null, self._nz_interior, self._maj_coeff_griffon, self._sub_coeff_griffon,
-                                   self._sup_coeff_griffon, self._mcoeff_griffon, self._ncoeff_griffon, self._x,
-                                   self._include_enthalpy_flux, self._include_variable_cp, False, rhs)
+        self._griffon.flamelet_rhs(state_interior, self._pressure, self._state_oxy, self._state_fuel,
+                                   *self._heat_loss_args(), self._nz_interior, self._maj_coeff_griffon,
+                                   self._sub_coeff_griffon, self._sup_coeff_griffon, self._mcoeff_griffon,
+                                   self._ncoeff_griffon, self._x, self._include_enthalpy_flux, self._include_variable_cp,
+                                   self._scale_heat_loss_by_temp_range, rhs)
         return rhs
 
-    def _adiabatic_jac(self, state_interior, scale_and_offset=False, prefactor=0.):
+    def _flamelet_jac(self, state_interior, scale_and_offset=False, prefactor=0.):
+        # the reference passes use_scaled_heat_loss=False to the adiabatic Jacobian wrappers, where it has no effect
         values = np.zeros(self._jac_nelements_griffon)
         null = np.zeros(1)
-        self._griffon.flamelet_jacobian(state_interior, self._pressure, self._state_oxy, self._state_fuel, True, null,
-                                        null, null, null, self._nz_interior, self._maj_coeff_griffon,
+        self._griffon.flamelet_jacobian(state_interior, self._pressure, self._state_oxy, self._state_fuel,
+                                        *self._heat_loss_args(), self._nz_interior, self._maj_coeff_griffon,
                                         self._sub_coeff_griffon, self._sup_coeff_griffon, self._mcoeff_griffon,
                                         self._ncoeff_griffon, self._x, False, 0., scale_and_offset, prefactor,
                                         self._rsopt, self._stopt, self._include_enthalpy_flux,
-                                        self._include_variable_cp, False, null, values)
+                                        self._include_variable_cp, self._scale_heat_loss_by_temp_range, null, values)
         return values
 
     def rhs(self, t, state_interior):
-        return self._adiabatic_rhs(t, state_interior)
+        return self._flamelet_rhs(t, state_interior)
 
     def jac(self, state_interior):
-        return self._adiabatic_jac(state_interior)
+        return self._flamelet_jac(state_interior)
 
     def _jac_and_expeig(self, state_interior, diffterm):
         """Jacobian values and the per-point explosive-eigenvalue estimate max(0, max Re(lambda) - diffterm)
         of each chemistry block (flamelet.py:1597-1600 with compute_eigenvalues=True)."""
         values = np.zeros(self._jac_nelements_griffon)
         expeig = np.zeros(self._n_dof)
-        null = np.zeros(1)
-        self._griffon.flamelet_jacobian(state_interior, self._pressure, self._state_oxy, self._state_fuel, True, null,
-                                        null, null, null, self._nz_interior, self._maj_coeff_griffon,
+        self._griffon.flamelet_jacobian(state_interior, self._pressure, self._state_oxy, self._state_fuel,
+                                        *self._heat_loss_args(), self._nz_interior, self._maj_coeff_griffon,
                                         self._sub_coeff_griffon, self._sup_coeff_griffon, self._mcoeff_griffon,
                                         self._ncoeff_griffon, self._x, True, diffterm, False, 0.,
                                         self._rsopt, self._stopt, self._include_enthalpy_flux,
-                                        self._include_variable_cp, False, expeig, values)
+                                        self._include_variable_cp, self._scale_heat_loss_by_temp_range, expeig, values)
         return values, expeig
 
     def eigenvalue_bound(self, state_interior, diffterm):
@@ -424,7 +484,7 @@ class Flamelet:
     # ---------------- transient integration (flamelet.py:1046-1237) ----------------
     def _setup_block_thomas(self, t, state_interior, prefactor):
         """prefactor * J - I, block-Thomas factored in place (flamelet.py:756-762)"""
-        self._jacobian_values = self._adiabatic_jac(state_interior, scale_and_offset=True, prefactor=prefactor)
+        self._jacobian_values = self._flamelet_jac(state_interior, scale_and_offset=True, prefactor=prefactor)
         self._factorize(self._jacobian_values)
 
     def _solve_block_thomas(self, residual):
@@ -450,7 +510,7 @@ class Flamelet:
             state[state < 0.] = 0.
             return state
 
-        out = odesolve(self._adiabatic_rhs, self._current_state, initial_time=self._current_time,
+        out = odesolve(self._flamelet_rhs, self._current_state, initial_time=self._current_time,
                        stop_at_time=stop_at_time, stop_at_steady=stop_at_steady, stop_criteria=stop_criteria,
                        minimum_time_step_count=minimum_time_step_count, post_step_callback=post_step,
                        method=stepper, step_size=control, linear_setup=self._setup_block_thomas,
@@ -631,6 +691,9 @@ class FlameletBatch:
             if (f._include_enthalpy_flux, f._include_variable_cp) != (f0._include_enthalpy_flux,
                                                                       f0._include_variable_cp):
                 raise ValueError('flamelets in a batch share the enthalpy-flux and variable-cp options')
+            if (f._heat_transfer, f._scale_heat_loss_by_temp_range) != (f0._heat_transfer,
+                                                                        f0._scale_heat_loss_by_temp_range):
+                raise ValueError('flamelets in a batch share the heat transfer setting and its scaling option')
         self._lib = lib()
         self._mech = mech
         self._ns = mech.n_species
@@ -646,6 +709,11 @@ class FlameletBatch:
                                                  int(f0._include_enthalpy_flux), int(f0._include_variable_cp),
                                                  ctypes.byref(h)))
         self._h = h
+        if f0._heat_transfer == 'nonadiabatic':
+            hl = [np.ascontiguousarray(np.concatenate([getattr(f, a) for f in self._flamelets]))
+                  for a in ('_T_conv', '_h_conv', '_T_rad', '_h_rad')]
+            check(self._lib.gf_flamelet_batch_set_heat_loss(h, *[dptr(a) for a in hl],
+                                                            int(f0._scale_heat_loss_by_temp_range)))
         self.set_state(np.concatenate([f._initial_state for f in self._flamelets]))
 
     def __del__(self):

@@ -121,6 +121,23 @@ def flamelet_fixture(name, mech, nzi, seed, variants=((0, 0), (0, 1), (1, 0), (1
         out['rhs_%d%d' % (ef, vcp)] = ref.flamelet_rhs(state, p, oxy, fuel, nzi, st, chi, incl_ef=ef, incl_vcp=vcp)
         out['jac_%d%d' % (ef, vcp)] = ref.flamelet_jacobian(state, p, oxy, fuel, nzi, st, chi, incl_ef=ef,
                                                             incl_vcp=vcp)[0]
+    # nonadiabatic: convection and radiation per interior point (seeded), plain and scaled by the temperature
+    # range, with the enthalpy-flux and variable-cp terms on; heat_loss in the C++ order (T_conv, h_conv, T_rad,
+    # h_rad)
+    hrng = np.random.default_rng(seed + 1000)
+    Tc = 300. + 200. * hrng.random(nzi)
+    hc = 10. ** hrng.uniform(6., 8., nzi)
+    Tr = 300. + 100. * hrng.random(nzi)
+    hr = hrng.uniform(0.1, 0.9, nzi)
+    out['hl_Tc'], out['hl_hc'], out['hl_Tr'], out['hl_hr'] = Tc, hc, Tr, hr
+    for sc in (0, 1):
+        out['rhs_hl%d' % sc] = ref.flamelet_rhs(state, p, oxy, fuel, nzi, st, chi, heat_loss=(Tc, hc, Tr, hr),
+                                                incl_ef=1, incl_vcp=1, scaled_hl=sc)
+        out['jac_hl%d' % sc] = ref.flamelet_jacobian(state, p, oxy, fuel, nzi, st, chi, heat_loss=(Tc, hc, Tr, hr),
+                                                     incl_ef=1, incl_vcp=1, scaled_hl=sc)[0]
+    out['jac_hl1_so'] = ref.flamelet_jacobian(state, p, oxy, fuel, nzi, st, chi, heat_loss=(Tc, hc, Tr, hr),
+                                              incl_ef=1, incl_vcp=1, scaled_hl=1, scale_and_offset=True,
+                                              prefactor=-2.5e-4)[0]
     out['jac_so'] = ref.flamelet_jacobian(state, p, oxy, fuel, nzi, st, chi, scale_and_offset=True,
                                           prefactor=-2.5e-4)[0]
     jac_eig, expeig = ref.flamelet_jacobian(state, p, oxy, fuel, nzi, st, chi, compute_eig=True, diffterm=4. * 5.)
@@ -212,11 +229,14 @@ def main(which=None):
         reactor_fixture('reactors_53.npz', synthetic.mechanism_53(), 3, seed=22, max_edge=2)
     if which == 'reactors':
         return
-    chem_fixture('chem_c1.npz', synthetic.mechanism_c1(), 48, seed=11)
-    chem_fixture('chem_53.npz', synthetic.mechanism_53(), 12, seed=12, max_edge=8)
+    if which in (None, 'chem'):
+        chem_fixture('chem_c1.npz', synthetic.mechanism_c1(), 48, seed=11)
+        chem_fixture('chem_53.npz', synthetic.mechanism_53(), 12, seed=12, max_edge=8)
+    if which == 'chem':
+        return
     flamelet_fixture('flamelet_c1.npz', synthetic.mechanism_c1(), 12, seed=13)
     flamelet_fixture('flamelet_53.npz', synthetic.mechanism_53(), 5, seed=14, variants=((0, 0), (1, 1)))
 
 
 if __name__ == '__main__':
-    main(sys.argv[1] if len(sys.argv) > 1 else None)
+    main(sys.argv[1] if len(sys.argv) > 1 else None)  # None (all), 'reactors', 'chem' or 'flamelets'

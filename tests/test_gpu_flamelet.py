@@ -50,6 +50,8 @@ def test_flamelet_dropins_match_reference_fixtures(fx, mech, request):
     null = np.zeros(1)
     st = f['state']
     for key in f.files:
+        if 'hl' in key:
+            continue
         if key.startswith('rhs_'):
             ef, vcp = int(key[4]), int(key[5])
             out = np.zeros(nv)
@@ -65,6 +67,24 @@ def test_flamelet_dropins_match_reference_fixtures(fx, mech, request):
             g.flamelet_jacobian(st, float(f['p']), f['oxy'], f['fuel'], True, null, null, null, null, nzi, cm, cs, cu,
                                 mc, nc, f['chi'], False, 0., so, -2.5e-4 if so else 0., 0, 0, bool(ef), bool(vcp),
                                 False, null, out)
+            e = _jac_err(out, f[key], nzi, ns)
+            say(key, 'err %.3e' % e)
+            assert e <= RTOL
+    # nonadiabatic, through the reference's argument order (T_convection, T_radiation, h_convection, h_radiation)
+    Tc, hc, Tr, hr = f['hl_Tc'], f['hl_hc'], f['hl_Tr'], f['hl_hr']
+    for sc in (0, 1):
+        out = np.zeros(nv)
+        g.flamelet_rhs(st, float(f['p']), f['oxy'], f['fuel'], False, Tc, Tr, hc, hr, nzi, cm, cs, cu, mc, nc,
+                       f['chi'], True, True, bool(sc), out)
+        e = _rows_err(out, f['rhs_hl%d' % sc], ns)
+        say('rhs_hl%d row err %.3e' % (sc, e))
+        assert e <= RTOL
+        for so in ((False, True) if sc else (False,)):
+            out = np.zeros(nel)
+            g.flamelet_jacobian(st, float(f['p']), f['oxy'], f['fuel'], False, Tc, Tr, hc, hr, nzi, cm, cs, cu, mc,
+                                nc, f['chi'], False, 0., so, -2.5e-4 if so else 0., 0, 0, True, True, bool(sc), null,
+                                out)
+            key = 'jac_hl%d' % sc + ('_so' if so else '')
             e = _jac_err(out, f[key], nzi, ns)
             say(key, 'err %.3e' % e)
             assert e <= RTOL

@@ -259,3 +259,48 @@ def test_time_stepping_stage_bounds(mech_c1_data):
                                             max_time_step=1e4, maximum_time_step_count=60, raise_on_failure=False)
     say('  cap: steps', steps.tolist(), 'stop', list(fb.transient_stop))
     assert not conv.any() and list(fb.transient_stop) == ['step limit', 'step limit'] and steps.tolist() == [61, 61]
+
+
+@pytest.mark.parametrize('case', ['convection', 'scaled'])
+def test_nonadiabatic_batch_steady_states_match_oracle(mech_c1_data, oracle_lib, case):
+    """Nonadiabatic flamelets (convection + radiation heat loss, flamelet_kernels.cpp:1121-1138, 1290-1320) solved
+    by the batched pseudo-transient continuation, FP64 and FP32 block LU, against the oracle's single-flamelet
+    solves: plain heat loss with scalar parameters, and heat loss scaled by the temperature range with per-point
+    parameter arrays."""
+    from spitfire_amd.flamelet import Flamelet, FlameletBatch, FlameletSpec
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from oracle.flamelet_oracle import OracleFlamelet
+    m = ChemicalMechanismSpec(mech_c1_data)
+    ns = m.n_species
+    air, fuel = m.stream(stp_air=True), m.stream('TPY', (300., 101325., 'H2:1 N2:1'))
+    chis, n = [3., 30.], 24
+
+    def spec(chi):
+        if case == 'convection':
+            hl = dict(convection_temperature=300., radiation_temperature=300., convection_coefficient=1e5,
+                      radiative_emissivity=0.5)
+        else:
+            hl = dict(convection_temperature=300., radiation_temperature=np.full(n - 2, 350.),
+                      convection_coefficient=1e8 * (1. + np.linspace(0., 1., n - 2)),
+                      radiative_emissivity=np.full(n - 2, 0.3), scale_heat_loss_by_temp_range=True)
+        return FlameletSpec(mech_spec=m, oxy_stream=air, fuel_stream=fuel, grid_points=n, stoich_dissipation_rate=chi,
+                            heat_transfer='nonadiabatic', **hl)
+
+    refs = []
+    for chi in chis:
+        o = OracleFlamelet(spec(chi))
+        _, it, ok, _ = o.steady_solve_psitc(tolerance=1e-9)
+        a = OracleFlamelet(FlameletSpec(mech_spec=m, oxy_stream=air, fuel_stream=fuel, grid_points=n,
+                                        stoich_dissipation_rate=chi))
+        a.steady_solve_psitc(tolerance=1e-9)
+        say('nonadiabatic %s chi %g: oracle %d iterations, Tmax %.1f (adiabatic %.1f)'
+            % (case, chi, it, o.current_temperature.max(), a.current_temperature.max()))
+        assert ok and a.current_temperature.max() - o.current_temperature.max() > 50.
+        refs.append(o.current_interior_state)
+    for prec in (64, 32):
+        fb = FlameletBatch([Flamelet(spec(c)) for c in chis])
+        states, conv, its = fb.steady_solve_psitc(tolerance=1e-9, lu_precision=prec)
+        for k, (g, ref) in enumerate(zip(states, refs)):
+            e = _scaled(g, ref, ns)
+            say('  fp%d flamelet %d: %d iterations, max scaled diff to the oracle %.3e' % (prec, k, its[k], e))
+            assert conv[k] and e <= 1e-8

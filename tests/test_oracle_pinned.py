@@ -113,12 +113,23 @@ def test_oracle_flamelet_matches_reference_fixtures(fx, mech, request, oracle_li
     assert np.array_equal(rows, f['rows']) and np.array_equal(cols, f['cols'])
     args = (f['state'], float(f['p']), f['oxy'], f['fuel'], nzi, st, f['chi'])
     for key in f.files:
-        if key.startswith('rhs_'):
+        if key.startswith('rhs_') and 'hl' not in key:
             ef, vcp = int(key[4]), int(key[5])
             assert rel_err(o.flamelet_rhs(*args, incl_ef=ef, incl_vcp=vcp), f[key], 1e-300) <= 1e-13
-        if key.startswith('jac_') and key != 'jac_so':
+        if key.startswith('jac_') and key != 'jac_so' and 'hl' not in key:
             ef, vcp = int(key[4]), int(key[5])
             assert rel_err(o.flamelet_jacobian(*args, incl_ef=ef, incl_vcp=vcp)[0], f[key], 1e-300) <= 1e-13
+    # nonadiabatic (convection + radiation, plain and scaled by the temperature range; C++ argument order)
+    hl = (f['hl_Tc'], f['hl_hc'], f['hl_Tr'], f['hl_hr'])
+    for sc in (0, 1):
+        kw = dict(heat_loss=hl, incl_ef=1, incl_vcp=1, scaled_hl=sc)
+        assert rel_err(o.flamelet_rhs(*args, **kw), f['rhs_hl%d' % sc], 1e-300) <= 1e-13
+        assert rel_err(o.flamelet_jacobian(*args, **kw)[0], f['jac_hl%d' % sc], 1e-300) <= 1e-13
+    so = o.flamelet_jacobian(*args, heat_loss=hl, incl_ef=1, incl_vcp=1, scaled_hl=1, scale_and_offset=True,
+                             prefactor=-2.5e-4)[0]
+    assert rel_err(so, f['jac_hl1_so'], 1e-300) <= 1e-13
+    # the heat loss moves the temperature rows by far more than the tolerance (the check is not vacuous)
+    assert np.max(np.abs(f['rhs_hl0'] - f['rhs_11'])[::ns] / np.abs(f['rhs_11'][::ns])) > 1e-6
     so = o.flamelet_jacobian(*args, scale_and_offset=True, prefactor=-2.5e-4)[0]
     assert rel_err(so, f['jac_so'], 1e-300) <= 1e-13
     # eigenvalue-based pseudo-time scale: numpy (LAPACK dgeev) on the same blocks as the reference

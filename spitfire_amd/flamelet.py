@@ -493,13 +493,16 @@ class Flamelet:
     def integrate(self, stop_at_time=None, stop_at_steady=None, stop_criteria=None, first_time_step=1.e-6,
                   max_time_step=1.e-3, minimum_time_step_count=40, transient_tolerance=1.e-10,
                   maximum_steps_per_jacobian=10, nonlinear_solve_tolerance=1.e-12, save_first_and_last_only=False,
-                  return_info=False):
+                  return_info=False, write_log=False, log_rate=100, linear_solver='block thomas',
+                  print_exception_on_failure=False):
         """ESDIRK64 with a frozen-Jacobian Newton solver, PI step control and block-Thomas linear solves of
         dt*gamma*J - I (flamelet.py:1046-1213, linear_solver='block thomas'); negative entries are clipped after
         every step. Starts from the current state and time and advances them. Returns a Library over (time,
         mixture fraction) of temperature, pressure and mass fractions, boundary streams included
         (flamelet.py:1187-1211)."""
         from .time_integration import KennedyCarpenterS6P4Q3, PIController, SimpleNewtonSolver, odesolve
+        if linear_solver.lower() != 'block thomas':
+            raise ValueError('linear solver ' + linear_solver + ' is not available; use block thomas')
         w = 1. / self._variable_scales
         newton = SimpleNewtonSolver(evaluate_jacobian_every_iter=False, norm_weighting=w,
                                     tolerance=nonlinear_solve_tolerance)
@@ -537,6 +540,16 @@ class Flamelet:
     def _has_ignited(self, state, delta_temperature_ignition):
         ne = self._n_equations
         return np.max(state[::ne] - self._initial_state[::ne]) > delta_temperature_ignition
+
+    def integrate_for_heat_loss(self, temperature_tolerance=0.05, steady_tolerance=1.e-4, **kwargs):
+        """integrate until max(state) <= (1 + temperature_tolerance) max(T_oxy, T_fuel) or steady
+        (flamelet.py:1263-1288; the reference tests the largest entry of the whole state vector)"""
+        T_bc_max = max(self._oxy_stream.T, self._fuel_stream.T)
+
+        def stop(t, state, residual, *args, **kw):
+            return np.max(state) < (1. + temperature_tolerance) * T_bc_max or residual < steady_tolerance
+
+        return self.integrate(stop_criteria=stop, **kwargs)
 
     def integrate_to_steady_after_ignition(self, steady_tolerance=1.e-4, delta_temperature_ignition=400., **kwargs):
         """integrate until steady after the temperature rose by delta_temperature_ignition somewhere

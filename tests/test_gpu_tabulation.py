@@ -155,3 +155,67 @@ def test_explosive_mode_analysis_vs_oracle(mech_53_data, configuration):
     ei = np.array([lib['cema-ei1 T']] + [lib['cema-ei1 ' + s] for s in m.species_names[:-1]])
     assert np.all(ei >= 0.) and np.all(ei.sum(axis=0) <= 1. + 1e-12)
     assert 'cema-lexp2' in lib and 'cema-ei2 T' in lib
+
+
+def _check_library_gold(name, lib, m, rtol, atol):
+    """every property of the reference's library file except viscosity (no transport here), and its dimensions"""
+    import os
+    from spitfire_amd.library import Library
+    gold = Library.load_from_file(os.path.join(os.path.dirname(__file__), 'golden', 'ref_%s_gold.pkl' % name))
+    assert lib.dim_names == gold.dim_names and lib.shape == gold.shape, (lib.dim_names, lib.shape, gold.shape)
+    for d, gd in zip(lib.dims, gold.dims):
+        np.testing.assert_allclose(d.values, gd.values, rtol=rtol, atol=atol, err_msg=d.name)
+    worst = 0.
+    for p in gold.props:
+        if p == 'viscosity':
+            continue
+        np.testing.assert_allclose(lib[p], gold[p], rtol=rtol, atol=atol, err_msg=p)
+        worst = max(worst, float(np.max(np.abs(lib[p] - gold[p]) / (atol + rtol * np.abs(gold[p])))))
+    print('  %s: largest |difference| / (atol + rtol |gold|) = %.3e' % (name, worst), flush=True)
+
+
+@pytest.mark.parametrize('name, rtol, atol', [('adiabatic_unreacted', 1e-7, 1e-14),
+                                              ('adiabatic_burke_schumann', 3e-3, 1e-7),
+                                              ('nonadiabatic_defect_burke_schumann', 3e-3, 1e-7)])
+def test_unstrained_libraries_match_reference_gold(name, rtol, atol):
+    """The reference's unreacted, Burke-Schumann and Burke-Schumann enthalpy-defect libraries
+    (tests/tabulation/<name>: h2-burke, air at 1200 K, H2 at 300 K, 34 points; the reference tests' tolerances, with
+    numpy's default rtol of 1e-7 where they give only atol)."""
+    from spitfire_amd import tabulation
+    m, spec, _ = slfm_setup()
+    print('library', name, flush=True)
+    build = {'adiabatic_unreacted': tabulation.build_unreacted_library,
+             'adiabatic_burke_schumann': tabulation.build_adiabatic_bs_library,
+             'nonadiabatic_defect_burke_schumann': tabulation.build_nonadiabatic_defect_bs_library}[name]
+    lib = _props(m, build(spec, verbose=False))
+    _check_library_gold(name, lib, m, rtol, atol)
+
+
+def test_nonadiabatic_defect_steady_slfm_library_matches_reference_gold():
+    """tests/tabulation/nonadiabatic_defect_steady_slfm: chi_st = logspace(0, 1, 4), rtol 2e-4, atol 1e-4; every
+    heat-loss step of the four dissipation rates solved in one batch."""
+    from spitfire_amd.tabulation import build_nonadiabatic_defect_steady_slfm_library
+    m, spec, _ = slfm_setup()
+    print('nonadiabatic defect steady SLFM library', flush=True)
+    lib = build_nonadiabatic_defect_steady_slfm_library(spec, verbose=False, diss_rate_values=np.logspace(0, 1, 4),
+                                                        diss_rate_log_scaled=True,
+                                                        integration_args={'transient_tolerance': 1e-10})
+    _check_library_gold('nonadiabatic_defect_steady_slfm', _props(m, lib), m, 2e-4, 1e-4)
+
+
+def test_nonadiabatic_defect_transient_slfm_library_matches_reference_gold():
+    """tests/tabulation/nonadiabatic_defect_transient_slfm: burke-hydrogen mechanism, chi_st = logspace(0, 1, 4),
+    transient tolerance 1e-10, rtol 2e-4, atol 1e-4."""
+    import os
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from spitfire_amd.tabulation import build_nonadiabatic_defect_transient_slfm_library
+    m = ChemicalMechanismSpec(cantera_input=os.path.join(os.path.dirname(__file__), 'golden', 'burke-hydrogen.yaml'),
+                              group_name='gas')
+    air = m.stream(stp_air=True)
+    air.TP = 1200., 101325.
+    fuel = m.stream('TPY', (300., 101325., 'H2:1'))
+    spec = {'mech_spec': m, 'oxy_stream': air, 'fuel_stream': fuel, 'grid_points': 34}
+    print('nonadiabatic defect transient SLFM library', flush=True)
+    lib = build_nonadiabatic_defect_transient_slfm_library(spec, verbose=True, diss_rate_values=np.logspace(0, 1, 4),
+                                                           integration_args={'transient_tolerance': 1e-10})
+    _check_library_gold('nonadiabatic_defect_transient_slfm', _props(m, lib), m, 2e-4, 1e-4)

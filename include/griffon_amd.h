@@ -284,6 +284,10 @@ int gf_flamelet_batch_destroy(gf_flamelet_batch *b);
  * make the batch adiabatic again (its initial setting). */
 int gf_flamelet_batch_set_heat_loss(gf_flamelet_batch *b, const double *T_conv, const double *h_conv,
                                     const double *T_rad, const double *h_rad, int use_scaled_heat_loss);
+/* Inverse Lewis numbers of every flamelet (host array [nfl][ns], flamelet f's at inv_lewis + f*ns; entry l scales
+ * the diffusion stencil of equation l, the invLewisNumbers argument of flamelet_stencils, griffon.pyx /
+ * flamelet_kernels.cpp:31-48). Null restores unity (the batch's initial setting). */
+int gf_flamelet_batch_set_lewis_numbers(gf_flamelet_batch *b, const double *inv_lewis);
 /* interior states, point-major per flamelet, concatenated: sum(nzi)*ns doubles; set_state also sets the
  * initial guess the restarts return to */
 int gf_flamelet_batch_set_state(gf_flamelet_batch *b, const double *interior);

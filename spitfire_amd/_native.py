@@ -84,6 +84,7 @@ SIGNATURES = {
     'gf_flamelet_batch_create': [_V, _int, _I, _dbl, _D, _D, _D, _D, _int, _int, ctypes.POINTER(_V)],
     'gf_flamelet_batch_destroy': [_V],
     'gf_flamelet_batch_set_heat_loss': [_V, _D, _D, _D, _D, _int],
+    'gf_flamelet_batch_set_lewis_numbers': [_V, _D],
     'gf_flamelet_batch_set_state': [_V, _D],
     'gf_flamelet_batch_get_state': [_V, _D],
     'gf_flamelet_batch_rhs': [_V, _D],

@@ -62,6 +62,7 @@ struct gf_flamelet_batch {
          *dz = nullptr, *dp = nullptr, *fval = nullptr, *fres = nullptr, *fmin = nullptr, *fdiff = nullptr;
   float *D32 = nullptr;  // FP32 inverse diagonal blocks (mixed-precision path); L holds the FP64 ones
   double *hl = nullptr;  // heat-loss parameters [4][NI] (own allocation), null while adiabatic
+  double *ile = nullptr; // inverse Lewis numbers [B][ns] (own allocation), null for unity
   int *piv = nullptr, *dnzi = nullptr, *flag = nullptr, *gmin = nullptr, *sel = nullptr, *sys_nb = nullptr;
   long *dpt0 = nullptr, *ddof0 = nullptr, *dipt0 = nullptr, *dchi0 = nullptr, *dvoff = nullptr, *ddz0 = nullptr,
        *js_off = nullptr, *eb = nullptr, *eo = nullptr, *sys_v = nullptr, *sys_l = nullptr, *sys_p = nullptr,
@@ -594,7 +595,7 @@ int gf_flamelet_batch_create(gf_mech *m, int nfl, const int *nzi, double pressur
   v.NV = V;
   v.NJ = Jv;
   for (int f = 0; f < nfl; ++f) v.maxNJ = std::max(v.maxNJ, (long)ns * ((long)nzi[f] * ns + 2 * (nzi[f] - 1)));
-  T(launch_fb_stencils(v, b->dz, b->ddz0, b->cmaj, b->csub, b->csup, b->mco, b->nco, st));
+  T(launch_fb_stencils(v, b->dz, b->ddz0, nullptr, b->cmaj, b->csub, b->csup, b->mco, b->nco, st));
   if (rc == GF_OK && hipStreamSynchronize(st) != hipSuccess) rc = fail(GF_ERR_HIP, "stream synchronize failed");
   if (rc != GF_OK) {
     hipFree(b->dmem);
@@ -610,6 +611,7 @@ int gf_flamelet_batch_destroy(gf_flamelet_batch *b) {
   if (!b) return GF_OK;
   if (b->dmem) hipFree(b->dmem);
   if (b->hl) hipFree(b->hl);
+  if (b->ile) hipFree(b->ile);
   if (b->hpin) hipHostFree(b->hpin);
   delete b;
   return GF_OK;
@@ -637,6 +639,22 @@ int gf_flamelet_batch_set_heat_loss(gf_flamelet_batch *b, const double *T_conv, 
   b->view.adiabatic = 0;
   b->view.scaled_hl = use_scaled_heat_loss ? 1 : 0;
   return GF_OK;
+}
+
+int gf_flamelet_batch_set_lewis_numbers(gf_flamelet_batch *b, const double *inv_lewis) {
+  if (!b) return fail(GF_ERR_ARG, "null flamelet batch");
+  const hipStream_t st = b->m->stream;
+  const double *dle = nullptr;
+  if (inv_lewis) {
+    const size_t n = (size_t)b->B * b->ns;
+    if (!b->ile && hipMalloc(&b->ile, n * sizeof(double)) != hipSuccess)
+      return fail(GF_ERR_HIP, "hipMalloc failed for the Lewis numbers");
+    if (hipMemcpyAsync(b->ile, inv_lewis, n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+      return fail(GF_ERR_HIP, "host-to-device copy failed");
+    dle = b->ile;
+  }
+  GF_TRY(launch_fb_stencils(b->view, b->dz, b->ddz0, dle, b->cmaj, b->csub, b->csup, b->mco, b->nco, st));
+  return hipStreamSynchronize(st) == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "stream synchronize failed");
 }
 
 int gf_flamelet_batch_set_state(gf_flamelet_batch *b, const double *interior) {

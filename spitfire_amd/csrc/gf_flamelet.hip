@@ -250,9 +250,10 @@ __device__ FlameletIO fb_io(const FlameletBatchView &b, int f, const double *X) 
   return io;
 }
 
-// mixture-fraction stencils of every flamelet (unity Lewis numbers), dz of flamelet f at dz + dz0[f]
-__global__ void fb_stencils_kernel(FlameletBatchView b, const double *dz, const long *dz0, double *cmaj, double *csub,
-                                   double *csup, double *mco, double *nco) {
+// mixture-fraction stencils of every flamelet, dz of flamelet f at dz + dz0[f]; equation l of flamelet f is
+// scaled by invLe[f * ns + l] (flamelet_kernels.cpp:31-48), unity when invLe is null
+__global__ void fb_stencils_kernel(FlameletBatchView b, const double *dz, const long *dz0, const double *invLe,
+                                   double *cmaj, double *csub, double *csup, double *mco, double *nco) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= b.NV) return;
   const int f = fb_find(b.dof0, b.B, e);
@@ -261,9 +262,10 @@ __global__ void fb_stencils_kernel(FlameletBatchView b, const double *dz, const 
   const double *d = dz + dz0[f];
   const double *chi = b.chi + b.chi0[f];
   const double dzt = d[i] + d[i + 1];
-  cmaj[e] = -chi[1 + i] / (d[i] * d[i + 1]);
-  csub[e] = chi[1 + i] / (dzt * d[i]);
-  csup[e] = chi[1 + i] / (dzt * d[i + 1]);
+  const double ile = invLe ? invLe[(long)f * b.ns + l] : 1.;
+  cmaj[e] = -chi[1 + i] / (d[i] * d[i + 1]) * ile;
+  csub[e] = chi[1 + i] / (dzt * d[i]) * ile;
+  csup[e] = chi[1 + i] / (dzt * d[i + 1]) * ile;
   if (l == 0) {
     nco[b.ipt0[f] + i] = 1 / (d[i] + d[i + 1]);
     mco[b.ipt0[f] + i] = -nco[b.ipt0[f] + i];
@@ -468,9 +470,9 @@ int launch_scale_and_add_diagonal(double *v, double a, const double *dg, double 
     return GF_OK;                                                                               \
   } while (0)
 
-int launch_fb_stencils(const FlameletBatchView &b, const double *dz, const long *dz0, double *cmaj, double *csub,
-                       double *csup, double *mco, double *nco, hipStream_t st) {
-  FB_LAUNCH(fb_stencils_kernel, b.NV, b, dz, dz0, cmaj, csub, csup, mco, nco);
+int launch_fb_stencils(const FlameletBatchView &b, const double *dz, const long *dz0, const double *invLe,
+                       double *cmaj, double *csub, double *csup, double *mco, double *nco, hipStream_t st) {
+  FB_LAUNCH(fb_stencils_kernel, b.NV, b, dz, dz0, invLe, cmaj, csub, csup, mco, nco);
 }
 int launch_fb_rhs(const FlameletBatchView &b, const double *X, const int *flag, double *R, hipStream_t st) {
   FB_LAUNCH(fb_rhs_kernel, b.NV, b, X, flag, R);

@@ -235,8 +235,8 @@ struct FlameletBatchView {
   long NV = 0, NJ = 0;  // total unknowns, total Jacobian entries
   long maxNJ = 0;       // Jacobian entries of the largest flamelet
 };
-int launch_fb_stencils(const FlameletBatchView &b, const double *dz, const long *dz0, double *cmaj, double *csub,
-                       double *csup, double *mco, double *nco, hipStream_t st);
+int launch_fb_stencils(const FlameletBatchView &b, const double *dz, const long *dz0, const double *invLe,
+                       double *cmaj, double *csub, double *csup, double *mco, double *nco, hipStream_t st);
 int launch_fb_rhs(const FlameletBatchView &b, const double *X, const int *flag, double *R, hipStream_t st);
 // mode 0: diag(1/ds) - J (pseudo-transient, vals = ds); 1: -J (Newton); 2: prefactor * J - I (implicit time
 // stepping, vals = per-flamelet prefactors, or the scalar prefactor when null)

@@ -722,6 +722,9 @@ class FlameletBatch:
                                                  int(f0._include_enthalpy_flux), int(f0._include_variable_cp),
                                                  ctypes.byref(h)))
         self._h = h
+        ile = np.ascontiguousarray(np.concatenate([1. / f._lewis_numbers for f in self._flamelets]))
+        if np.any(ile != 1.):
+            check(self._lib.gf_flamelet_batch_set_lewis_numbers(h, dptr(ile)))
         if f0._heat_transfer == 'nonadiabatic':
             hl = [np.ascontiguousarray(np.concatenate([getattr(f, a) for f in self._flamelets]))
                   for a in ('_T_conv', '_h_conv', '_T_rad', '_h_rad')]

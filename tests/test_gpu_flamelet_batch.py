@@ -304,3 +304,44 @@ def test_nonadiabatic_batch_steady_states_match_oracle(mech_c1_data, oracle_lib,
             e = _scaled(g, ref, ns)
             say('  fp%d flamelet %d: %d iterations, max scaled diff to the oracle %.3e' % (prec, k, its[k], e))
             assert conv[k] and e <= 1e-8
+
+
+def test_lewis_numbers_batch_matches_oracle(mech_c1_data, oracle_lib):
+    """Non-unity Lewis numbers, different for each flamelet (each Flamelet's _lewis_numbers, the array the
+    reference passes to flamelet_stencils, flamelet_kernels.cpp:31-48): the batched right-hand side and the batched
+    pseudo-transient steady states against the oracle flamelets with the same numbers."""
+    from spitfire_amd.flamelet import FlameletBatch
+    from spitfire_amd.mechanism import ChemicalMechanismSpec
+    from oracle.flamelet_oracle import OracleFlamelet
+    m = ChemicalMechanismSpec(mech_c1_data)
+    ns = m.n_species
+    chis, npts = [3., 30.], [20, 26]
+    rng = np.random.default_rng(4)
+    les = [np.r_[1., rng.uniform(0.3, 2.0, ns - 1)] for _ in chis]
+
+    def with_lewis(fl):
+        for f, le in zip(fl, les):
+            f._lewis_numbers = le.copy()
+            f._setup_stencils()
+        return fl
+
+    say('Lewis numbers, batch rhs and psitc vs oracle')
+    fb = FlameletBatch(with_lewis(_flamelets(m, chis, npts)))
+    got = fb.split(fb.rhs())
+    orc = with_lewis(_flamelets(m, chis, npts, OracleFlamelet))
+    for f, g in zip(orc, got):
+        ref = f.rhs(0., f.initial_interior_state)
+        e = np.max(np.abs(g - ref).reshape(-1, ns).max(axis=1) / np.maximum(np.abs(ref).reshape(-1, ns).max(axis=1),
+                                                                          1e-300))
+        say('  rhs nzi %d row err %.3e' % (f.n_interior_points, e))
+        assert e <= 1e-12
+    # the Lewis numbers move the right-hand side far beyond that tolerance (the comparison is not vacuous)
+    unity = _flamelets(m, chis, npts, OracleFlamelet)[0].rhs(0., orc[0].initial_interior_state)
+    d = np.abs(unity - orc[0].rhs(0., orc[0].initial_interior_state)).reshape(-1, ns).max(axis=1)
+    assert np.max(d / np.abs(unity).reshape(-1, ns).max(axis=1)) > 1e-6
+    states, conv, its = fb.steady_solve_psitc(tolerance=1e-9)
+    for k, f in enumerate(orc):
+        _, it, ok, _ = f.steady_solve_psitc(tolerance=1e-9)
+        e = _scaled(states[k], f.current_interior_state, ns)
+        say('  flamelet %d: batch %d iterations, oracle %s, max scaled diff %.3e' % (k, its[k], it, e))
+        assert conv[k] and ok and e <= 1e-8

@@ -13,7 +13,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, 'liboracle.so')
+# GF_ORACLE_SO selects another build of the same source (the sanitizer build, oracle/Makefile target asan)
+ORACLE_SO = os.environ.get('GF_ORACLE_SO') or os.path.join(HERE, 'liboracle.so')
 REF_SO = os.path.join(HERE, '_ref', 'libgriffon_ref.so')
 
 _D = ctypes.POINTER(ctypes.c_double)

@@ -456,6 +456,8 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
     if (JAC) {
       __syncthreads();
       // ---------------- P3: Jacobian columns (wave = column, lane = row; lane ns = the last species) -------
+      // Every multiply-add here is written as an explicit fma or a rounded product (__dmul_rn), so the compiler's
+      // contraction choices cannot differ between launch shapes and every shape gives bitwise the same Jacobian.
       int nact = 0;
       double iR[S], iC[S], iRC[S], roT[S], rhs0[S], rmw[S], T00[S];
       double *jb[S];  // Jacobian block of each state (nullptr: none)
@@ -480,14 +482,26 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
       }
       if (a.jac) {
         constexpr int NW = THREADS / 64;
+        // per-row values of every state, loaded once for all the columns this wave assembles
+        const int iiL = lane - 1;
+        const bool rowL = lane >= 1 && lane <= ns;
+        double rA[S], rB[S], rH[S], rP[S];
+        for (int s = 0; s < S; ++s) {
+          const double *sp = sbase + s * stride;
+          rA[s] = rowL ? sp[A_ * SP_STRIDE + iiL] : 0.;
+          rB[s] = rowL ? sp[B_ * SP_STRIDE + iiL] : 0.;
+          rH[s] = rowL ? sp[H_ * SP_STRIDE + iiL] : 0.;
+          rP[s] = lane < ns ? sp[PR_ * SP_STRIDE + lane] : 0.;
+        }
         for (int c = wave; c < ns; c += NW) {
           double val[S];
           if (c == 0) {
             for (int s = 0; s < S; ++s) {
               const double *sp = sbase + s * stride;
               const double *pr = sp + PR_ * SP_STRIDE;
-              val[s] = (lane == 0) ? T00[s] - roT[s] * pr[0]
-                                   : (lane < ns ? sp[WS1_ * SP_STRIDE + lane - 1] * iR[s] - roT[s] * pr[lane] : 0.);
+              val[s] = (lane == 0) ? fma(-roT[s], pr[0], T00[s])
+                                   : (lane < ns ? fma(-roT[s], pr[lane], __dmul_rn(sp[WS1_ * SP_STRIDE + lane - 1], iR[s]))
+                                                : 0.);
             }
           } else {
             const int j = c - 1;
@@ -513,28 +527,29 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
                   vbv[s] = vb[s * stride + sb];
                 }
                 for (int s = 0; s < S; ++s) {
-                  full[s] += ca * va[s];
-                  full[s] += cb * vbv[s];
+                  full[s] = fma(ca, va[s], full[s]);
+                  full[s] = fma(cb, vbv[s], full[s]);
                 }
               }
               if (t < t1) {
                 const uint32_t rec = exx[t];
                 const int slot = (int)(rec & 0xFFFFu);
                 const double coef = -(double)(int8_t)(rec >> 16) * Mi;
-                for (int s = 0; s < S; ++s) full[s] += coef * vb[s * stride + slot];
+                for (int s = 0; s < S; ++s) full[s] = fma(coef, vb[s * stride + slot], full[s]);
               }
-              for (int s = 0; s < S; ++s) {
-                const double *sp = sbase + s * stride;
-                full[s] += dj * sp[A_ * SP_STRIDE + ii] + sp[B_ * SP_STRIDE + ii];
-              }
+              for (int s = 0; s < S; ++s) full[s] += fma(dj, rA[s], rB[s]);
             }
             for (int s = 0; s < S; ++s) {
               const double *sp = sbase + s * stride;
               // temperature row: -(1/(rho cp)) sum_k h_k dw_k/dY_j over every species (isobaric_reactor_kernels.cpp:58-98)
-              const double hs = wave_sum(row ? sp[H_ * SP_STRIDE + ii] * full[s] : 0.);
-              val[s] = (lane == 0) ? -iRC[s] * hs - rhs0[s] * (sp[CPI_ * SP_STRIDE + j] - sp[CPI_ * SP_STRIDE + nm]) * iC[s] +
-                                         rmw[s] * dj * sp[PR_ * SP_STRIDE]
-                                   : (lane < ns ? iR[s] * full[s] + rmw[s] * dj * sp[PR_ * SP_STRIDE + lane] : 0.);
+              const double hs = wave_sum(row ? __dmul_rn(rH[s], full[s]) : 0.);
+              const double rd = __dmul_rn(rmw[s], dj);
+              val[s] = (lane == 0)
+                           ? fma(-iRC[s], hs, __dmul_rn(rd, sp[PR_ * SP_STRIDE]) -
+                                                  __dmul_rn(__dmul_rn(rhs0[s], sp[CPI_ * SP_STRIDE + j] -
+                                                                                   sp[CPI_ * SP_STRIDE + nm]),
+                                                            iC[s]))
+                           : (lane < ns ? fma(iR[s], full[s], __dmul_rn(rd, rP[s])) : 0.);
             }
           }
           if (lane < ns)

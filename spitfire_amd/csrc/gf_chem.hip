@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
       // Every multiply-add here is written as an explicit fma or a rounded product (__dmul_rn), so the compiler's
       // contraction choices cannot differ between launch shapes and every shape gives bitwise the same Jacobian.
       int nact = 0;
-      double iR[S], iC[S], iRC[S], roT[S], rhs0[S], rmw[S], T00[S];
+      double iR[S], iRC[S], ric[S], rmw[S];  // 1/rho, 1/(rho cp), rhs0/cp, -rho M of each state
       double *jb[S];  // Jacobian block of each state (nullptr: none)
       for (int s = 0; s < S; ++s) {
         jb[s] = nullptr;
@@ -473,12 +473,9 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
         }
         const double *sc = sbase + s * stride + NSA * SP_STRIDE;
         iR[s] = 1. / sc[S_RHO];
-        iC[s] = 1. / sc[S_CP];
         iRC[s] = 1. / (sc[S_RHO] * sc[S_CP]);
-        roT[s] = sc[S_RHO] / sc[S_T];
-        rhs0[s] = sc[S_RHS0];
+        ric[s] = sc[S_RHS0] / sc[S_CP];
         rmw[s] = -sc[S_RHO] * sc[S_MMW];
-        T00[s] = -iRC[s] * (sc[S_HS1] + sc[S_WCP]) - rhs0[s] * sc[S_CPST] * iC[s];
       }
       if (a.jac) {
         constexpr int NW = THREADS / 64;
@@ -498,9 +495,12 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
           if (c == 0) {
             for (int s = 0; s < S; ++s) {
               const double *sp = sbase + s * stride;
+              const double *sc = sp + NSA * SP_STRIDE;
               const double *pr = sp + PR_ * SP_STRIDE;
-              val[s] = (lane == 0) ? fma(-roT[s], pr[0], T00[s])
-                                   : (lane < ns ? fma(-roT[s], pr[lane], __dmul_rn(sp[WS1_ * SP_STRIDE + lane - 1], iR[s]))
+              const double roT = sc[S_RHO] / sc[S_T];
+              const double T00 = fma(-iRC[s], sc[S_HS1] + sc[S_WCP], -__dmul_rn(ric[s], sc[S_CPST]));
+              val[s] = (lane == 0) ? fma(-roT, pr[0], T00)
+                                   : (lane < ns ? fma(-roT, pr[lane], __dmul_rn(sp[WS1_ * SP_STRIDE + lane - 1], iR[s]))
                                                 : 0.);
             }
           } else {
@@ -546,9 +546,7 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
               const double rd = __dmul_rn(rmw[s], dj);
               val[s] = (lane == 0)
                            ? fma(-iRC[s], hs, __dmul_rn(rd, sp[PR_ * SP_STRIDE]) -
-                                                  __dmul_rn(__dmul_rn(rhs0[s], sp[CPI_ * SP_STRIDE + j] -
-                                                                                   sp[CPI_ * SP_STRIDE + nm]),
-                                                            iC[s]))
+                                                  __dmul_rn(ric[s], sp[CPI_ * SP_STRIDE + j] - sp[CPI_ * SP_STRIDE + nm]))
                            : (lane < ns ? fma(iR[s], full[s], __dmul_rn(rd, rP[s])) : 0.);
             }
           }

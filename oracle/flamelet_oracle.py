@@ -21,6 +21,11 @@ class OracleFlamelet(Flamelet):
         (self._maj_coeff_griffon, self._sub_coeff_griffon, self._sup_coeff_griffon, self._mcoeff_griffon,
          self._ncoeff_griffon) = st
 
+    def _jac_indices(self):
+        if getattr(self, '_jac_indices_griffon', None) is None:
+            self._jac_indices_griffon = self._cpu.flamelet_jac_indices(self._nz_interior)
+        return self._jac_indices_griffon
+
     def _oracle_heat_loss(self):
         """(T_conv, h_conv, T_rad, h_rad), the oracle's (C++) order, or None when adiabatic"""
         if self._heat_transfer == 'adiabatic':

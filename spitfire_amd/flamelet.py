@@ -61,7 +61,7 @@ class FlameletSpec:
                  convection_temperature=None, radiation_temperature=None, convection_coefficient=None,
                  radiative_emissivity=None, scale_heat_loss_by_temp_range=False, scale_convection_by_dissipation=False,
                  use_linear_ref_temp_profile=False, include_enthalpy_flux=True, include_variable_cp=True,
-                 rates_sensitivity_type='dense', sensitivity_transform_type='exact'):
+                 rates_sensitivity_type='dense', sensitivity_transform_type='exact', library_slice=None):
         if heat_transfer not in ('adiabatic', 'nonadiabatic'):
             raise ValueError('Flamelet specifications: Bad heat_transfer argument detected: ' + str(heat_transfer) +
                              "; acceptable values: ['adiabatic', 'nonadiabatic']")
@@ -90,6 +90,35 @@ class FlameletSpec:
         self.include_variable_cp = include_variable_cp
         self.rates_sensitivity_type = rates_sensitivity_type
         self.sensitivity_transform_type = sensitivity_transform_type
+        if library_slice is not None:
+            self._from_library_slice(library_slice)
+
+    def _from_library_slice(self, lib):
+        """mechanism, streams, grid and initial state from a one-dimensional library over mixture fraction (trivial
+        extra dimensions are squeezed; flamelet.py:135-180)"""
+        from .library import Library
+        shape = lib.shape
+        if len(shape) < 1 or any(n > 1 for n in shape[1:]):
+            raise ValueError('Error in Flamelet construction from library_slice. The library must be one-dimensional, '
+                             'with mixture fraction as the first dimension (shape = %s).' % (shape,))
+        if len(shape) > 1:
+            lib = Library.squeeze(lib)
+        if 'mixture_fraction' not in lib.dims[0].name:
+            raise ValueError('Error in Flamelet construction from library_slice. The library provided does not appear '
+                             'to have mixture fraction as the first dimension.')
+        m = lib.extra_attributes['mech_spec']
+        self.mech_spec = m
+        names = m.species_names
+        p = lib['pressure'][0]
+        self.oxy_stream = m.stream('TPY', (lib['temperature'][0], p, [lib['mass fraction ' + s][0] for s in names]))
+        self.fuel_stream = m.stream('TPY', (lib['temperature'][-1], p, [lib['mass fraction ' + s][-1] for s in names]))
+        self.grid = lib.mixture_fraction_values
+        self.grid_points = self.grid_type = self.grid_cluster_intensity = self.grid_cluster_point = None
+        ic = np.zeros((self.grid.size - 2, m.n_species))
+        ic[:, 0] = lib['temperature'][1:-1]
+        for i, s in enumerate(names[:-1]):
+            ic[:, 1 + i] = lib['mass fraction ' + s][1:-1]
+        self.initial_condition = ic.ravel()
 
 
 class Flamelet:
@@ -286,6 +315,40 @@ class Flamelet:
 
     def jac(self, state_interior):
         return self._flamelet_jac(state_interior)
+
+    def _jac_indices(self):
+        if getattr(self, '_jac_indices_griffon', None) is None:
+            rows = np.zeros(self._jac_nelements_griffon, dtype=np.int32)
+            cols = np.zeros(self._jac_nelements_griffon, dtype=np.int32)
+            self._griffon.flamelet_jac_indices(self._nz_interior, rows, cols)
+            self._jac_indices_griffon = (rows, cols)
+        return self._jac_indices_griffon
+
+    def jac_csc(self, state_interior):
+        """the Jacobian as a scipy CSC matrix (flamelet.py:897-898)"""
+        from scipy.sparse import csc_matrix
+        return csc_matrix((self._flamelet_jac(state_interior), self._jac_indices()))
+
+    def jac_and_eig(self, state_interior, diffterm=0.):
+        """Jacobian values and the per-point explosive-eigenvalue estimate (flamelet.py:900-901; the reference's
+        method passes no diffterm to its helper)"""
+        return self._jac_and_expeig(state_interior, diffterm)
+
+    @classmethod
+    def _uniform_grid(cls, grid_points):
+        return uniform_grid(grid_points)
+
+    @classmethod
+    def _clustered_grid(cls, grid_points, grid_cluster_point, grid_cluster_intensity=6.):
+        return clustered_grid(grid_points, grid_cluster_point, grid_cluster_intensity)
+
+    @classmethod
+    def make_clustered_grid(cls, grid_points, grid_cluster_point, grid_cluster_intensity=6.):
+        return clustered_grid(grid_points, grid_cluster_point, grid_cluster_intensity)[0]
+
+    @classmethod
+    def _compute_dissipation_rate(cls, mixture_fraction, max_dissipation_rate, form='Peters'):
+        return compute_dissipation_rate(mixture_fraction, max_dissipation_rate, form)
 
     def _jac_and_expeig(self, state_interior, diffterm):
         """Jacobian values and the per-point explosive-eigenvalue estimate max(0, max Re(lambda) - diffterm)
@@ -487,6 +550,18 @@ class Flamelet:
         self._jacobian_values = self._flamelet_jac(state_interior, scale_and_offset=True, prefactor=prefactor)
         self._factorize(self._jacobian_values)
 
+    def _setup_superlu(self, t, state_interior, prefactor):
+        """prefactor * J - I as a CSC matrix, factored by SuperLU (flamelet.py:750-754)"""
+        from scipy.sparse import csc_matrix
+        from scipy.sparse.linalg import splu
+        jac = csc_matrix((self._flamelet_jac(state_interior, scale_and_offset=True, prefactor=prefactor),
+                          self._jac_indices()))
+        jac.eliminate_zeros()
+        self._linear_inverse_operator = splu(jac)
+
+    def _solve_superlu(self, residual):
+        return self._linear_inverse_operator.solve(residual), 1, True
+
     def _solve_block_thomas(self, residual):
         return self._solve(self._jacobian_values, residual)
 
@@ -501,8 +576,12 @@ class Flamelet:
         mixture fraction) of temperature, pressure and mass fractions, boundary streams included
         (flamelet.py:1187-1211)."""
         from .time_integration import KennedyCarpenterS6P4Q3, PIController, SimpleNewtonSolver, odesolve
-        if linear_solver.lower() != 'block thomas':
-            raise ValueError('linear solver ' + linear_solver + ' is not available; use block thomas')
+        if linear_solver.lower() == 'superlu':
+            setup, solve = self._setup_superlu, self._solve_superlu
+        elif linear_solver.lower() == 'block thomas':
+            setup, solve = self._setup_block_thomas, self._solve_block_thomas
+        else:
+            raise ValueError('linear solver ' + linear_solver + ' is invalid, must be superlu or block thomas')
         w = 1. / self._variable_scales
         newton = SimpleNewtonSolver(evaluate_jacobian_every_iter=False, norm_weighting=w,
                                     tolerance=nonlinear_solve_tolerance)
@@ -516,8 +595,8 @@ class Flamelet:
         out = odesolve(self._flamelet_rhs, self._current_state, initial_time=self._current_time,
                        stop_at_time=stop_at_time, stop_at_steady=stop_at_steady, stop_criteria=stop_criteria,
                        minimum_time_step_count=minimum_time_step_count, post_step_callback=post_step,
-                       method=stepper, step_size=control, linear_setup=self._setup_block_thomas,
-                       linear_solve=self._solve_block_thomas, linear_setup_rate=maximum_steps_per_jacobian,
+                       method=stepper, step_size=control, linear_setup=setup,
+                       linear_solve=solve, linear_setup_rate=maximum_steps_per_jacobian,
                        norm_weighting=w, save_each_step=not save_first_and_last_only, return_info=True)
         if save_first_and_last_only:
             state, t, _, info = out

@@ -22,6 +22,7 @@
 #include "gf_wave.h"
 #include "gf_twist.h"
 
+#include <atomic>
 #include <utility>
 
 namespace gfa {
@@ -29,6 +30,19 @@ namespace gfa {
 namespace {
 
 using wave::lane_value;
+
+// Compute units of the current device, looked up once per device.
+int compute_units() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int n = cache[dev].load(std::memory_order_relaxed);
+  if (n <= 0) {
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cache[dev].store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
 
 constexpr int BW = 64;
 constexpr int BT_THREADS = 256;
@@ -369,7 +383,8 @@ int launch_btd_solve_t(int nsys, const int *nblocks, int bs, const double *value
 }
 
 template <typename T>
-using TwistFactorFn = void (*)(int, const int *, const double *, const long *, T *, const long *, hipStream_t);
+using TwistFactorFn = void (*)(int, const int *, const double *, const long *, T *, const long *, bool,
+                               hipStream_t);
 
 template <typename T, int... I>
 constexpr TwistFactorFn<T> twist_factor_table_entry(int bs, std::integer_sequence<int, I...>) {
@@ -382,7 +397,13 @@ int launch_btd_inv_factor(int nsys, const int *nblocks, int bs, const double *va
                           const long *doff, hipStream_t st) {
   if (nsys <= 0) return GF_OK;
   if (bs < 1 || bs > 64) return fail(GF_ERR_UNSUPPORTED, "block size must be in 1..64");
-  twist_factor_table_entry<T>(bs, std::make_integer_sequence<int, 64>{})(nsys, nblocks, values, voff, dinv, doff, st);
+  // Two waves per chain shorten every pivot step, but their barriers cost issue slots: measured on 53x53
+  // blocks, the split kernel wins up to 512 systems (chains of 62..254 blocks: 7.5 against 11.2 ms) and loses
+  // from 1024 systems on (single blocks: 0.105 against 0.057 ms; chains: 13.2 against 11.9 ms), i.e. once
+  // the systems alone give every SIMD (four per compute unit) a chain.
+  const bool split = nsys < 4 * compute_units();
+  twist_factor_table_entry<T>(bs, std::make_integer_sequence<int, 64>{})(nsys, nblocks, values, voff, dinv, doff,
+                                                                          split, st);
   if (hipGetLastError() != hipSuccess) return fail(GF_ERR_HIP, "btd inverse factor launch failed");
   return GF_OK;
 }

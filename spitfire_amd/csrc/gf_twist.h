@@ -7,10 +7,12 @@
 
 namespace gfa {
 
-// Launches btd_twist_factor_kernel<T, N> (one workgroup of two waves per system). Defined for N in 1..64 and
-// T in {double, float} by the gf_twist_*.hip units.
+// Launches the twisted factorisation for block size N: with `split` (and N >= 12) the column-split kernel,
+// two waves per chain, whose pivot steps are shorter; otherwise one wave per chain, which wastes no issue
+// slots on barriers once the batch alone fills every SIMD. Defined for N in 1..64 and T in {double, float} by
+// the gf_twist_*.hip units.
 template <typename T, int N>
 void launch_twist_factor(int nsys, const int *nblocks, const double *values, const long *voff, T *dinv,
-                         const long *doff, hipStream_t st);
+                         const long *doff, bool split, hipStream_t st);
 
 }  // namespace gfa

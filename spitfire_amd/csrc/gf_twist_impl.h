@@ -29,8 +29,9 @@ __device__ __forceinline__ void reg_gj_inverse(T (&x)[N], int &rowid, int &colid
   bool used = lane >= N;
   for (int j = 0; j < N; ++j) {
     const T f = x[0];
+    const T rf = T(1) / f;  // every lane's reciprocal, formed beside the pivot search instead of after it
     const int p = wave_first_argmax(fabs(f), !used);  // N - j unused lanes remain, so p >= 0
-    const T ipiv = T(1) / lane_value(f, p);
+    const T ipiv = lane_value(rf, p);
     const bool isp = lane == p;
     const T g = isp ? -ipiv : f * ipiv;   // multiplier of the broadcast pivot row
     const T keep = isp ? T(0) : T(1);     // the pivot row itself becomes (pivot row) / pivot
@@ -141,12 +142,215 @@ __global__ void __launch_bounds__(128) btd_twist_factor_kernel(const int *nblock
   }
 }
 
+// ---- Column-split variant: W waves share each block inverse ----
+//
+// The single-wave inverse above is a serial chain of N pivot steps, each about 300 instructions of one wave
+// (52 column updates, each two readlanes and two FP64 operations, plus the pivot search and the division),
+// and a flamelet system has only one such chain per half. Splitting the columns of the block over W waves
+// divides the update work of a step by W: wave w holds slots w*C .. w*C + C - 1 (C = ceil(N / W)) of every
+// lane's row. Step j is pivoted by the wave that holds slot j; it publishes the multipliers g (one per lane)
+// and the pivot lane through LDS, and after one workgroup barrier every wave updates its own slots. The
+// arithmetic of every entry is the one of the single-wave routine (same pivot, same g, same fma), so the
+// inverse is the same.
+//
+// All waves compute a pivot candidate on their own slot of step j (only the owner publishes it): that work
+// sits in the same basic block as the updates of step j - 1, where the compiler can interleave it, and the
+// other waves would otherwise wait at the barrier. The multipliers are double-buffered on the step parity
+// `ph` (carried across calls), so one barrier per step suffices: the buffer written at step j was last read
+// at step j - 2, before every wave passed the barrier of step j - 1.
+template <int N>
+constexpr int twist_waves() {
+  return N >= 12 ? 2 : 1;
+}
+
+template <typename T, int N, int W>
+__device__ __forceinline__ void split_gj_inverse(T (&x)[(N + W - 1) / W], int &rowid, int &colid, int lane, int w,
+                                                 T *gb, int *pb, int &ph) {
+  constexpr int C = (N + W - 1) / W;
+  bool used = lane >= N;
+  for (int ww = 0; ww < W; ++ww) {
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int j = ww * C + k;
+      if (j < N) {
+        {
+          const T f = x[k];
+          const T rf = T(1) / f;  // every lane's reciprocal, formed beside the pivot search instead of after it
+          const int p = wave_first_argmax(fabs(f), !used);  // N - j unused lanes remain, so p >= 0
+          const T ipiv = lane_value(rf, p);
+          const T g = lane == p ? -ipiv : f * ipiv;
+          if (w == ww) {
+            gb[ph * 64 + lane] = g;
+            if (lane == 0) pb[ph] = p;
+          }
+        }
+        __syncthreads();
+        const int p = __builtin_amdgcn_readfirstlane(pb[ph]);
+        const T g = gb[ph * 64 + lane];
+        ph ^= 1;
+        const bool isp = lane == p;
+        const T keep = isp ? T(0) : T(1);
+#pragma unroll
+        for (int c = 0; c < C; ++c) x[c] = fma(-g, lane_value(x[c], p), x[c] * keep);
+        if (w == ww) x[k] = -g;  // the pivot column: -f / pivot, and 1 / pivot on the pivot lane
+        const int cj = __builtin_amdgcn_readlane(colid, j);
+        const int rp = __builtin_amdgcn_readlane(rowid, p);
+        if (isp) {
+          rowid = cj;
+          used = true;
+        }
+        if (lane == j) colid = rp;
+      }
+    }
+  }
+}
+
+// Slot k of wave w is logical slot w*C + k; slots at or beyond N are padding and stay zero.
+template <typename T, int N, int W>
+__device__ __forceinline__ void split_load_block(T (&x)[(N + W - 1) / W], const double *Vi, int &rowid, int &colid,
+                                                 int lane, int w) {
+  constexpr int C = (N + W - 1) / W;
+  rowid = lane;
+  colid = lane;
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    const int s = w * C + k;
+    x[k] = (lane < N && s < N) ? (T)Vi[s * N + lane] : T(0);
+  }
+}
+
+// The identity: the block a chain inverts while the other chain inverts a real one, so that both run the same
+// number of barriers.
+template <typename T, int N, int W>
+__device__ __forceinline__ void split_load_identity(T (&x)[(N + W - 1) / W], int &rowid, int &colid, int lane,
+                                                    int w) {
+  constexpr int C = (N + W - 1) / W;
+  rowid = lane;
+  colid = lane;
+#pragma unroll
+  for (int k = 0; k < C; ++k) x[k] = (lane == w * C + k && lane < N) ? T(1) : T(0);
+}
+
+template <typename T, int N, int W>
+__device__ __forceinline__ void split_schur(T (&x)[(N + W - 1) / W], const double *Vi, const double *rs,
+                                            const double *cs, int rowid, int colid, int lane, int w) {
+  constexpr int C = (N + W - 1) / W;
+  const bool act = lane < N;
+  const double r = act ? rs[rowid] : 0.;
+  const double csl = act ? cs[lane] : 0.;  // lane q holds cs[q]
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    const int s = w * C + k;
+    if (s < N) {
+      const int c = __builtin_amdgcn_readlane(colid, s);
+      const double cc = lane_value(csl, c);
+      if (act) x[k] = (T)(Vi[c * N + rowid] - r * cc * (double)x[k]);
+    }
+  }
+}
+
+template <typename T, int N, int W>
+__device__ __forceinline__ void split_store_block(const T (&x)[(N + W - 1) / W], T *Di, int rowid, int colid,
+                                                  int lane, int w) {
+  constexpr int C = (N + W - 1) / W;
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    const int s = w * C + k;
+    if (s < N) {
+      const int c = __builtin_amdgcn_readlane(colid, s);
+      if (lane < N) Di[c * N + rowid] = x[k];
+    }
+  }
+}
+
+// The twisted factorisation of btd_twist_factor_kernel with W waves per chain (2W per workgroup: waves
+// 0 .. W-1 run the top chain, W .. 2W-1 the bottom chain). Both chains run max(len) inverses so that every
+// wave passes the same barriers: the shorter chain starts with one inverse of the identity, which it
+// discards (it comes first, so the last real inverse stays in registers for the middle block). The middle
+// block is inverted by the top chain alone.
+template <typename T, int N, int W>
+__global__ void __launch_bounds__(128 * W) btd_twist_factor_split_kernel(const int *nblocks, const double *values,
+                                                                         const long *voff, T *dinv,
+                                                                         const long *doff) {
+  constexpr int C = (N + W - 1) / W;
+  constexpr long NB2 = (long)N * N;
+  __shared__ double sb[N * N];
+  __shared__ T gbuf[2][2 * 64];  // [chain][parity * 64 + lane]
+  __shared__ int pbuf[2][2];     // [chain][parity]
+  const int sys = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int chain = wave / W;
+  const int w = wave - chain * W;
+  const bool top = chain == 0;
+  const int nb = nblocks[sys];
+  const int m = (nb - 1) / 2;
+  const double *V = values + voff[sys];
+  const double *sub = V + nb * NB2;
+  const double *sup = sub + (long)(nb - 1) * N;
+  T *Dg = dinv + doff[sys];
+  T *gb = gbuf[chain];
+  int *pb = pbuf[chain];
+  T x[C];
+  int rowid = lane, colid = lane, ph = 0;
+  const int len = top ? m : nb - 1 - m;
+  const int iters = nb - 1 - m;  // >= m
+  for (int t = 0; t < iters; ++t) {
+    const int tt = t - (iters - len);  // < 0: the leading identity of the shorter chain
+    const int i = top ? tt : nb - 1 - tt;
+    if (tt < 0) split_load_identity<T, N, W>(x, rowid, colid, lane, w);
+    else if (tt == 0) split_load_block<T, N, W>(x, V + i * NB2, rowid, colid, lane, w);
+    else if (top) split_schur<T, N, W>(x, V + i * NB2, sub + (i - 1) * N, sup + (i - 1) * N, rowid, colid, lane, w);
+    else split_schur<T, N, W>(x, V + i * NB2, sup + i * N, sub + i * N, rowid, colid, lane, w);
+    split_gj_inverse<T, N, W>(x, rowid, colid, lane, w, gb, pb, ph);
+    if (tt >= 0) split_store_block<T, N, W>(x, Dg + i * NB2, rowid, colid, lane, w);
+  }
+  const bool below = nb - 1 - m > 0;
+  if (!top && below) {
+    const double r = lane < N ? sup[m * N + rowid] : 0.;
+    const double csl = lane < N ? sub[m * N + lane] : 0.;
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int s = w * C + k;
+      if (s < N) {
+        const int c = __builtin_amdgcn_readlane(colid, s);
+        const double cc = lane_value(csl, c);
+        if (lane < N) sb[c * N + rowid] = r * (double)x[k] * cc;
+      }
+    }
+  }
+  __syncthreads();
+  // The bottom chain is done. Its waves end here, and the barriers of the middle inverse wait only on the
+  // waves still running (s_barrier counts the surviving waves of a workgroup).
+  if (!top) return;
+  if (m == 0) split_load_block<T, N, W>(x, V + m * NB2, rowid, colid, lane, w);
+  else split_schur<T, N, W>(x, V + m * NB2, sub + (m - 1) * N, sup + (m - 1) * N, rowid, colid, lane, w);
+  if (below) {
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int s = w * C + k;
+      if (s < N) {
+        const int c = __builtin_amdgcn_readlane(colid, s);
+        if (lane < N) x[k] = (T)((double)x[k] - sb[c * N + rowid]);
+      }
+    }
+  }
+  split_gj_inverse<T, N, W>(x, rowid, colid, lane, w, gb, pb, ph);
+  split_store_block<T, N, W>(x, Dg + m * NB2, rowid, colid, lane, w);
+}
+
 }  // namespace
 
 template <typename T, int N>
 void launch_twist_factor(int nsys, const int *nblocks, const double *values, const long *voff, T *dinv,
-                         const long *doff, hipStream_t st) {
-  hipLaunchKernelGGL((btd_twist_factor_kernel<T, N>), dim3(nsys), dim3(128), 0, st, nblocks, values, voff, dinv, doff);
+                         const long *doff, bool split, hipStream_t st) {
+  constexpr int W = twist_waves<N>();
+  if (W == 1 || !split)
+    hipLaunchKernelGGL((btd_twist_factor_kernel<T, N>), dim3(nsys), dim3(128), 0, st, nblocks, values, voff, dinv,
+                       doff);
+  else
+    hipLaunchKernelGGL((btd_twist_factor_split_kernel<T, N, (W > 1 ? W : 2)>), dim3(nsys), dim3(128 * W), 0, st,
+                       nblocks, values, voff, dinv, doff);
 }
 
 }  // namespace gfa
@@ -154,6 +358,6 @@ void launch_twist_factor(int nsys, const int *nblocks, const double *values, con
 // Explicit instantiations of the launcher (and so of its kernel) for block size N, FP64 and FP32 inverses.
 #define GF_TWIST(N)                                                                                        \
   template void gfa::launch_twist_factor<double, N>(int, const int *, const double *, const long *, double *, \
-                                                    const long *, hipStream_t);                            \
+                                                    const long *, bool, hipStream_t);                      \
   template void gfa::launch_twist_factor<float, N>(int, const int *, const double *, const long *, float *,   \
-                                                   const long *, hipStream_t);
+                                                   const long *, bool, hipStream_t);

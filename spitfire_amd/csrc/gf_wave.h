@@ -38,11 +38,13 @@ __device__ __forceinline__ double dpp_wave_sum(double v) {
 
 // First lane holding the largest a >= 0 among the candidate lanes (LAPACK idamax semantics); -1 if none.
 // Non-negative doubles order like their bit patterns, so the maximum is found on the high word, then on the
-// low word among the lanes that hold the maximal high word.
+// low word among the lanes that hold the maximal high word (skipped when only one lane holds it).
 __device__ __forceinline__ int wave_first_argmax(double a, bool cand) {
   const unsigned long long bits = cand ? (unsigned long long)__double_as_longlong(a) : 0ull;
   const unsigned hi = (unsigned)(bits >> 32), lo = (unsigned)bits;
   const unsigned mh = dpp_wave_max_u32(hi);
+  const unsigned long long top = __ballot(cand && hi == mh);
+  if (__popcll(top) == 1) return __ffsll((long long)top) - 1;  // the usual case: one lane holds the top word
   const unsigned ml = dpp_wave_max_u32(hi == mh ? lo : 0u);
   const unsigned long long hit = __ballot(cand && hi == mh && lo == ml);
   return hit ? __ffsll((long long)hit) - 1 : -1;

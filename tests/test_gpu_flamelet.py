@@ -153,7 +153,7 @@ def test_batched_btd_ragged_matches_oracle(mech_53_data, oracle_lib):
         assert e <= 1e-11
 
 
-@pytest.mark.parametrize('bs', [1, 5, 53, 64])
+@pytest.mark.parametrize('bs', [1, 5, 13, 33, 53, 64])
 @pytest.mark.parametrize('precision', [64, 32])
 def test_batched_btd_inverse_path_matches_oracle(bs, precision, mech_53_data, oracle_lib):
     """The throughput form of the block Thomas recurrence (inverse diagonal blocks, chain eliminated from both
@@ -202,6 +202,52 @@ def test_batched_btd_inverse_path_matches_oracle(bs, precision, mech_53_data, or
         e = np.max(np.abs(Xh[xoff[k]:xoff[k] + nb * bs] - xr)) / np.max(np.abs(xr))
         say('system', nb, 'blocks: err %.3e' % e)
         assert e <= tol
+
+
+@pytest.mark.parametrize('bs', [13, 53])
+@pytest.mark.parametrize('precision', [64, 32])
+def test_inverse_factor_kernels_agree_bitwise(bs, precision):
+    """The inverse factor has two kernels for block sizes >= 12: columns split over two waves per chain for
+    batches of fewer than four systems per compute unit, one wave per chain for larger batches. Both apply
+    the same pivots and the same operations to every entry, so the same systems factored in a small batch and
+    inside a batch large enough for the other kernel give bitwise the same inverse blocks."""
+    import torch
+    from spitfire_amd import _native
+    rng = np.random.default_rng(7 + bs)
+    sizes = [1, 2, 3, 6, 33]
+    vals = []
+    for nb in sizes:
+        v = rng.standard_normal(bs * (nb * bs + 2 * (nb - 1)))
+        for i in range(nb):
+            v[i * bs * bs:(i + 1) * bs * bs].reshape(bs, bs)[np.diag_indices(bs)] += 10. + 4. * np.sqrt(bs)
+        vals.append(v)
+    dev = torch.device('cuda:0')
+    ncu = torch.cuda.get_device_properties(dev).multi_processor_count
+    lib = _native.lib()
+    s = torch.cuda.current_stream().cuda_stream
+    fdt = torch.float64 if precision == 64 else torch.float32
+
+    def factor(reps):
+        sz = sizes * reps
+        vv = vals * reps
+        voff = np.cumsum([0] + [v.size for v in vv])[:-1]
+        doff = np.cumsum([0] + [nb * bs * bs for nb in sz])[:-1]
+        V = torch.tensor(np.concatenate(vv), device=dev)
+        D = torch.zeros(int(sum(nb * bs * bs for nb in sz)), dtype=fdt, device=dev)
+        nbt = torch.tensor(sz, dtype=torch.int32, device=dev)
+        vo, do = [torch.tensor(a.astype(np.int64), device=dev) for a in (voff, doff)]
+        _native.check(lib.gf_batch_btddod_inverse_factorize(len(sz), nbt.data_ptr(), bs, V.data_ptr(), vo.data_ptr(),
+                                                            D.data_ptr(), do.data_ptr(), precision, s))
+        torch.cuda.synchronize()
+        return D.cpu().numpy()
+
+    reps = -(-4 * ncu // len(sizes))  # at least four systems per compute unit
+    say('inverse factor kernels bs', bs, 'precision', precision, 'systems', len(sizes), 'and', len(sizes) * reps)
+    small, large = factor(1), factor(reps)
+    n1 = small.size
+    assert np.all(np.isfinite(small))
+    for r in range(reps):
+        assert np.array_equal(large[r * n1:(r + 1) * n1], small)
 
 
 @pytest.mark.parametrize('fx, mech', [('flamelet_c1.npz', 'mech_c1_data'), ('flamelet_53.npz', 'mech_53_data')])

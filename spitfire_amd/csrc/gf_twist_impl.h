@@ -24,20 +24,30 @@ using wave::wave_first_argmax;
 // and appends the transformed pivot column at x[N-1], so that all register indices are compile-time
 // constants and after N steps every column is back in its slot. The pivot row is broadcast through scalar
 // registers (readlane), so a step has no LDS traffic and no barrier.
+//
+// Row scaling is deferred: lane r keeps its row unscaled and a factor sig_r, the true row being sig_r times
+// the stored one. Every row is pivoted exactly once and has sig = 1 until then, so at the step that pivots
+// lane p (stored pivot value d):
+//   rows r != p:   x_r[c] -= g_r x_p[c] (c != k),  x_r[k] = -g_r,  with g_r = x_r[k] / d
+//   row p:         x_p[c] unchanged,  x_p[k] = 1,  sig_p = 1 / d
+// which is the usual step (row p divided by its pivot) with the division of row p left to the end: the update
+// of a slot is one fma, with no separate scaling of the pivot row. After N steps every row is multiplied by
+// its sig.
 template <typename T, int N>
 __device__ __forceinline__ void reg_gj_inverse(T (&x)[N], int &rowid, int &colid, int lane) {
   bool used = lane >= N;
+  T sig = T(1);
   for (int j = 0; j < N; ++j) {
     const T f = x[0];
     const T rf = T(1) / f;  // every lane's reciprocal, formed beside the pivot search instead of after it
     const int p = wave_first_argmax(fabs(f), !used);  // N - j unused lanes remain, so p >= 0
-    const T ipiv = lane_value(rf, p);
+    const T ipv = lane_value(rf, p);
     const bool isp = lane == p;
-    const T g = isp ? -ipiv : f * ipiv;   // multiplier of the broadcast pivot row
-    const T keep = isp ? T(0) : T(1);     // the pivot row itself becomes (pivot row) / pivot
+    const T g = isp ? T(0) : f * ipv;  // multiplier of the broadcast pivot row (none for the pivot row)
 #pragma unroll
-    for (int c = 1; c < N; ++c) x[c - 1] = fma(-g, lane_value(x[c], p), x[c] * keep);
-    x[N - 1] = -g;                        // the pivot column: -f / pivot, and 1 / pivot on the pivot lane
+    for (int c = 1; c < N; ++c) x[c - 1] = fma(-g, lane_value(x[c], p), x[c]);
+    x[N - 1] = isp ? T(1) : -g;  // the pivot column
+    if (isp) sig = ipv;
     const int cj = __builtin_amdgcn_readlane(colid, j);
     const int rp = __builtin_amdgcn_readlane(rowid, p);
     if (isp) {
@@ -46,6 +56,8 @@ __device__ __forceinline__ void reg_gj_inverse(T (&x)[N], int &rowid, int &colid
     }
     if (lane == j) colid = rp;
   }
+#pragma unroll
+  for (int c = 0; c < N; ++c) x[c] *= sig;
 }
 
 // Block i of the values in the identity layout (lane = row, slot = column).
@@ -148,10 +160,10 @@ __global__ void __launch_bounds__(128) btd_twist_factor_kernel(const int *nblock
 // (52 column updates, each two readlanes and two FP64 operations, plus the pivot search and the division),
 // and a flamelet system has only one such chain per half. Splitting the columns of the block over W waves
 // divides the update work of a step by W: wave w holds slots w*C .. w*C + C - 1 (C = ceil(N / W)) of every
-// lane's row. Step j is pivoted by the wave that holds slot j; it publishes the multipliers g (one per lane)
-// and the pivot lane through LDS, and after one workgroup barrier every wave updates its own slots. The
-// arithmetic of every entry is the one of the single-wave routine (same pivot, same g, same fma), so the
-// inverse is the same.
+// lane's row. Step j is pivoted by the wave that holds slot j; it publishes the multipliers g (one per lane;
+// the pivot lane's entry carries its scale 1 / d) and the pivot lane through LDS, and after one workgroup
+// barrier every wave updates its own slots. The arithmetic of every entry is the one of the single-wave
+// routine (same pivot, same g, same fma, same final scaling), so the inverse is bitwise the same.
 //
 // All waves compute a pivot candidate on their own slot of step j (only the owner publishes it): that work
 // sits in the same basic block as the updates of step j - 1, where the compiler can interleave it, and the
@@ -168,6 +180,7 @@ __device__ __forceinline__ void split_gj_inverse(T (&x)[(N + W - 1) / W], int &r
                                                  T *gb, int *pb, int &ph) {
   constexpr int C = (N + W - 1) / W;
   bool used = lane >= N;
+  T sig = T(1);
   for (int ww = 0; ww < W; ++ww) {
 #pragma unroll
     for (int k = 0; k < C; ++k) {
@@ -177,22 +190,22 @@ __device__ __forceinline__ void split_gj_inverse(T (&x)[(N + W - 1) / W], int &r
           const T f = x[k];
           const T rf = T(1) / f;  // every lane's reciprocal, formed beside the pivot search instead of after it
           const int p = wave_first_argmax(fabs(f), !used);  // N - j unused lanes remain, so p >= 0
-          const T ipiv = lane_value(rf, p);
-          const T g = lane == p ? -ipiv : f * ipiv;
+          const T ipv = lane_value(rf, p);
           if (w == ww) {
-            gb[ph * 64 + lane] = g;
+            gb[ph * 64 + lane] = lane == p ? ipv : f * ipv;  // g of each row; the pivot lane carries 1 / d
             if (lane == 0) pb[ph] = p;
           }
         }
         __syncthreads();
         const int p = __builtin_amdgcn_readfirstlane(pb[ph]);
-        const T g = gb[ph * 64 + lane];
+        const T gl = gb[ph * 64 + lane];
         ph ^= 1;
         const bool isp = lane == p;
-        const T keep = isp ? T(0) : T(1);
+        const T g = isp ? T(0) : gl;
 #pragma unroll
-        for (int c = 0; c < C; ++c) x[c] = fma(-g, lane_value(x[c], p), x[c] * keep);
-        if (w == ww) x[k] = -g;  // the pivot column: -f / pivot, and 1 / pivot on the pivot lane
+        for (int c = 0; c < C; ++c) x[c] = fma(-g, lane_value(x[c], p), x[c]);
+        if (w == ww) x[k] = isp ? T(1) : -g;  // the pivot column
+        if (isp) sig = gl;
         const int cj = __builtin_amdgcn_readlane(colid, j);
         const int rp = __builtin_amdgcn_readlane(rowid, p);
         if (isp) {
@@ -203,6 +216,8 @@ __device__ __forceinline__ void split_gj_inverse(T (&x)[(N + W - 1) / W], int &r
       }
     }
   }
+#pragma unroll
+  for (int c = 0; c < C; ++c) x[c] *= sig;
 }
 
 // Slot k of wave w is logical slot w*C + k; slots at or beyond N are padding and stay zero.
@@ -269,7 +284,7 @@ __device__ __forceinline__ void split_store_block(const T (&x)[(N + W - 1) / W],
 // discards (it comes first, so the last real inverse stays in registers for the middle block). The middle
 // block is inverted by the top chain alone.
 template <typename T, int N, int W>
-__global__ void __launch_bounds__(128 * W) btd_twist_factor_split_kernel(const int *nblocks, const double *values,
+__global__ void __launch_bounds__(128 * W) __attribute__((amdgpu_waves_per_eu(2))) btd_twist_factor_split_kernel(const int *nblocks, const double *values,
                                                                          const long *voff, T *dinv,
                                                                          const long *doff) {
   constexpr int C = (N + W - 1) / W;
@@ -295,6 +310,44 @@ __global__ void __launch_bounds__(128 * W) btd_twist_factor_split_kernel(const i
   int rowid = lane, colid = lane, ph = 0;
   const int len = top ? m : nb - 1 - m;
   const int iters = nb - 1 - m;  // >= m
+  // Cache warming: while a block is inverted, the chain's waves read one word of every 128-byte line of the
+  // block they update next (its values and the two scale vectors), so that its Schur update after this
+  // inverse finds them in cache instead of waiting on HBM. The loads are volatile (not removed or moved
+  // after the inverse) and their results are consumed only after it.
+  auto warm = [&](int t) -> unsigned {
+    int i = -1;
+    const double *rs = nullptr, *cs = nullptr;
+    if (t == iters) {
+      if (top) {
+        i = m;
+        if (m > 0) { rs = sub + (m - 1) * N; cs = sup + (m - 1) * N; }
+      }
+    } else {
+      const int tt = t - (iters - len);
+      if (tt >= 0) {
+        i = top ? tt : nb - 1 - tt;
+        if (tt > 0) {
+          if (top) { rs = sub + (i - 1) * N; cs = sup + (i - 1) * N; }
+          else { rs = sup + i * N; cs = sub + i * N; }
+        }
+      }
+    }
+    if (i < 0) return 0u;
+    constexpr int LB = (int)((NB2 * 8 + 127) / 128);  // lines of a block
+    constexpr int LV = (N * 8 + 127) / 128;           // lines of a scale vector
+    unsigned acc = 0u;
+    for (int u = w * 64 + lane; u < LB + 2 * LV; u += W * 64) {
+      const char *a;
+      if (u < LB) a = reinterpret_cast<const char *>(V + i * NB2) + min((long)u * 128, NB2 * 8 - 4);
+      else {
+        const double *vec = u < LB + LV ? rs : cs;
+        if (!vec) continue;
+        a = reinterpret_cast<const char *>(vec) + min((u - LB - (u < LB + LV ? 0 : LV)) * 128, N * 8 - 4);
+      }
+      acc += *reinterpret_cast<const volatile unsigned *>(a);
+    }
+    return acc;
+  };
   for (int t = 0; t < iters; ++t) {
     const int tt = t - (iters - len);  // < 0: the leading identity of the shorter chain
     const int i = top ? tt : nb - 1 - tt;
@@ -302,7 +355,9 @@ __global__ void __launch_bounds__(128 * W) btd_twist_factor_split_kernel(const i
     else if (tt == 0) split_load_block<T, N, W>(x, V + i * NB2, rowid, colid, lane, w);
     else if (top) split_schur<T, N, W>(x, V + i * NB2, sub + (i - 1) * N, sup + (i - 1) * N, rowid, colid, lane, w);
     else split_schur<T, N, W>(x, V + i * NB2, sup + i * N, sub + i * N, rowid, colid, lane, w);
+    const unsigned warmed = warm(t + 1);
     split_gj_inverse<T, N, W>(x, rowid, colid, lane, w, gb, pb, ph);
+    asm volatile("" ::"v"(warmed));
     if (tt >= 0) split_store_block<T, N, W>(x, Dg + i * NB2, rowid, colid, lane, w);
   }
   const bool below = nb - 1 - m > 0;

@@ -64,6 +64,41 @@ __device__ __forceinline__ double wave_sum(double v) {
                           __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
 
+// Swaps of 32-bit halves between two lane groups, applied to both words of a double: with 32, the upper 32
+// lanes of a with the lower 32 lanes of b; with 16, the odd 16-lane rows of a with the even rows of b.
+template <int W>
+__device__ __forceinline__ void lane_swap(double &a, double &b) {
+  const auto hi = W == 32 ? __builtin_amdgcn_permlane32_swap(__double2hiint(a), __double2hiint(b), false, false)
+                          : __builtin_amdgcn_permlane16_swap(__double2hiint(a), __double2hiint(b), false, false);
+  const auto lo = W == 32 ? __builtin_amdgcn_permlane32_swap(__double2loint(a), __double2loint(b), false, false)
+                          : __builtin_amdgcn_permlane16_swap(__double2loint(a), __double2loint(b), false, false);
+  a = __hiloint2double((int)hi[0], (int)lo[0]);
+  b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+// Sums over the 64 lanes of four vectors at once, returned to every lane. Lanes l and l + 32 are added
+// (two vectors per register after a 32-lane swap), then rows r and r + 1 (four vectors per register after a
+// 16-lane swap), then the 16 lanes of each row by a DPP scan. Each vector is summed by the same tree whatever
+// its position (every pairwise sum is commutative), so a vector's sum does not depend on what the other three
+// hold; about a third of the instructions of four separate wave sums.
+__device__ __forceinline__ void wave_sum4(const double (&v)[4], double (&out)[4]) {
+  double a0 = v[0], b0 = v[1], a1 = v[2], b1 = v[3];
+  lane_swap<32>(a0, b0);  // a0 = [v0 lanes 0-31 | v1 lanes 0-31], b0 = [v0 lanes 32-63 | v1 lanes 32-63]
+  lane_swap<32>(a1, b1);
+  double p = a0 + b0, q = a1 + b1;  // p: v0 in lanes 0-31, v1 in lanes 32-63; q likewise v2, v3
+  lane_swap<16>(p, q);              // p = rows (p0, q0, p2, q2), q = rows (p1, q1, p3, q3)
+  double t = p + q;                 // rows hold v0, v2, v1, v3
+  t = dpp_add<0x111, 0xF>(t);       // row_shr:1
+  t = dpp_add<0x112, 0xF>(t);       // row_shr:2
+  t = dpp_add<0x114, 0xF>(t);       // row_shr:4
+  t = dpp_add<0x118, 0xF>(t);       // row_shr:8: lane 15 of each row holds the row's sum
+  const int src[4] = {15, 47, 31, 63};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    out[k] = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(t), src[k]),
+                              __builtin_amdgcn_readlane(__double2loint(t), src[k]));
+}
+
 __device__ __forceinline__ double arrh(const double *k, double logT, double invT) {
   return k[0] * exp(k[1] * logT - k[2] * invT);
 }
@@ -99,6 +134,7 @@ __device__ __forceinline__ double troe_fcent(const DevRxn &R, double T, double i
 
 template <int S, int THREADS, bool JAC>
 __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, long groups_per_wg) {
+  static_assert(S >= 1 && S <= 4, "the temperature-row sums take at most four states at once");
   extern __shared__ double lds[];
   const int ns = mv.ns, nr = mv.nr, nm = ns - 1;
   const int lane = threadIdx.x & 63;
@@ -483,6 +519,8 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
         const int iiL = lane - 1;
         const bool rowL = lane >= 1 && lane <= ns;
         double rA[S], rB[S], rH[S], rP[S];
+        // the row's molecular weight (loaded here: in the column loop the Jacobian stores could alias it)
+        const double MiL = rowL ? mv.mw[iiL] : 0.;
         for (int s = 0; s < S; ++s) {
           const double *sp = sbase + s * stride;
           rA[s] = rowL ? sp[A_ * SP_STRIDE + iiL] : 0.;
@@ -512,7 +550,7 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
             for (int s = 0; s < S; ++s) full[s] = 0.;
             if (row) {
               for (int s = 0; s < S; ++s) full[s] = (ii == j) ? sbase[s * stride + DG_ * SP_STRIDE + j] : 0.;
-              const double Mi = mv.mw[ii];
+              const double Mi = MiL;
               const int ee = j * ns + ii;
               const int t0 = exo[ee], t1 = exo[ee + 1];
               const double *vb = sbase + NSA * SP_STRIDE + N_SCAL + 5 * nr;
@@ -539,10 +577,15 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
               }
               for (int s = 0; s < S; ++s) full[s] += fma(dj, rA[s], rB[s]);
             }
+            // temperature row: -(1/(rho cp)) sum_k h_k dw_k/dY_j over every species (isobaric_reactor_kernels.cpp:58-98),
+            // the S states' sums formed together (absent states are zero vectors)
+            double hv[4], hsum[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) hv[s] = (s < S && row) ? __dmul_rn(rH[s < S ? s : 0], full[s < S ? s : 0]) : 0.;
+            wave_sum4(hv, hsum);
             for (int s = 0; s < S; ++s) {
               const double *sp = sbase + s * stride;
-              // temperature row: -(1/(rho cp)) sum_k h_k dw_k/dY_j over every species (isobaric_reactor_kernels.cpp:58-98)
-              const double hs = wave_sum(row ? __dmul_rn(rH[s], full[s]) : 0.);
+              const double hs = hsum[s];
               const double rd = __dmul_rn(rmw[s], dj);
               val[s] = (lane == 0)
                            ? fma(-iRC[s], hs, __dmul_rn(rd, sp[PR_ * SP_STRIDE]) -

@@ -204,7 +204,7 @@ def test_batched_btd_inverse_path_matches_oracle(bs, precision, mech_53_data, or
         assert e <= tol
 
 
-@pytest.mark.parametrize('bs', [13, 53])
+@pytest.mark.parametrize('bs', [12, 13, 53, 64])
 @pytest.mark.parametrize('precision', [64, 32])
 def test_inverse_factor_kernels_agree_bitwise(bs, precision):
     """The inverse factor has two kernels for block sizes >= 12: columns split over two waves per chain for

@@ -155,6 +155,10 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
   const uint64_t *sgx = reinterpret_cast<const uint64_t *>(tb + mv.sgx_at);
   const uint16_t *exo = reinterpret_cast<const uint16_t *>(tb + mv.exo_at);
   const uint32_t *exx = reinterpret_cast<const uint32_t *>(tb + mv.exx_at);
+  const double *tbe = reinterpret_cast<const double *>(tb + mv.tbe_at);  // third-body lists and 1/M in LDS
+  const int *tbi = reinterpret_cast<const int *>(tb + mv.tbi_at);
+  const int *tbsl = reinterpret_cast<const int *>(tb + mv.tbs_at);
+  const double *imw = reinterpret_cast<const double *>(tb + mv.imw_at);
   double *sbase = lds + tcopy / 8;
   __syncthreads();
 
@@ -223,7 +227,7 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
 
       // third-body / falloff multiplier
       double tbsum = 0.;
-      for (int q = 0; q < R.n_tb; ++q) tbsum += mv.tb_eff[R.tb_off + q] * y[mv.tb_idx[R.tb_off + q]];
+      for (int q = 0; q < R.n_tb; ++q) tbsum += tbe[R.tb_off + q] * y[tbi[R.tb_off + q]];
 
       if (!JAC) {
         // production_rates formulation (chemistry_kernels.cpp:140-461)
@@ -254,12 +258,12 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
           }
           for (int j = 0; j < R.n_rc; ++j) {
             const int sidx = R.rc_idx[j];
-            k *= ipow3(y[sidx] * rho * mv.invmw[sidx], R.rc_st[j]);
+            k *= ipow3(y[sidx] * rho * imw[sidx], R.rc_st[j]);
           }
           if (R.reversible)
             for (int j = 0; j < R.n_pd; ++j) {
               const int sidx = R.pd_idx[j];
-              kr *= ipow3(y[sidx] * rho * mv.invmw[sidx], R.pd_st[j]);
+              kr *= ipow3(y[sidx] * rho * imw[sidx], R.pd_st[j]);
             }
         }
         rq[r] = k - kr;
@@ -351,13 +355,13 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
         double C[NRC_MAX];
         Rnet = kf;
         for (int j = 0; j < R.n_rc; ++j) {
-          C[j] = y[R.rc_idx[j]] * rho * mv.invmw[R.rc_idx[j]];
+          C[j] = y[R.rc_idx[j]] * rho * imw[R.rc_idx[j]];
           Rnet *= ipow3(C[j], R.rc_st[j]);
         }
         dRdrho = Rnet / ct * invM * R.sumRc;
         dRdT = Rnet * ksens;
         for (int j = 0; j < R.n_rc; ++j) {
-          double d = kf * rho * mv.invmw[R.rc_idx[j]] * R.rc_st[j] * (R.rc_st[j] == 1 ? 1. : ipow3(C[j], R.rc_st[j] - 1));
+          double d = kf * rho * imw[R.rc_idx[j]] * R.rc_st[j] * (R.rc_st[j] == 1 ? 1. : ipow3(C[j], R.rc_st[j] - 1));
           for (int l = 0; l < R.n_rc; ++l)
             if (l != j) d *= ipow3(C[l], R.rc_st[l]);
           if (R.rc_slot[j] >= 0) vr[R.rc_slot[j]] += Ctbaf * d;
@@ -374,14 +378,14 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
           double Cp[NPD_MAX];
           double Rr = kr;
           for (int j = 0; j < R.n_pd; ++j) {
-            Cp[j] = y[R.pd_idx[j]] * rho * mv.invmw[R.pd_idx[j]];
+            Cp[j] = y[R.pd_idx[j]] * rho * imw[R.pd_idx[j]];
             Rr *= ipow3(Cp[j], R.pd_st[j]);
           }
           Rnet -= Rr;
           dRdrho -= Rr / ct * invM * R.sumPd;
           dRdT -= Rr * (ksens + sb);
           for (int j = 0; j < R.n_pd; ++j) {
-            double d = kr * rho * mv.invmw[R.pd_idx[j]] * R.pd_st[j] * (R.pd_st[j] == 1 ? 1. : ipow3(Cp[j], R.pd_st[j] - 1));
+            double d = kr * rho * imw[R.pd_idx[j]] * R.pd_st[j] * (R.pd_st[j] == 1 ? 1. : ipow3(Cp[j], R.pd_st[j] - 1));
             for (int l = 0; l < R.n_pd; ++l)
               if (l != j) d *= ipow3(Cp[l], R.pd_st[l]);
             if (R.pd_slot[j] >= 0) vr[R.pd_slot[j]] -= Ctbaf * d;
@@ -391,8 +395,8 @@ __global__ void __launch_bounds__(THREADS) chem_kernel(MechView mv, ChemArgs a, 
       }
       double Elast = 0.;
       for (int q = 0; q < R.n_tb; ++q) {
-        const int slot = mv.tb_slot[R.tb_off + q];
-        const double e = mv.tb_eff[R.tb_off + q];
+        const int slot = tbsl[R.tb_off + q];
+        const double e = tbe[R.tb_off + q];
         if (slot >= 0) vr[slot] += Rnet * tbs * e;
         else if (q == R.tb_last) Elast = tbs * e;
       }

@@ -70,11 +70,14 @@ struct MechView {
   //   u32 sgx_off[ns+1]; u64 sgx[n_sgx]   species k <- reactions with k in their net stoichiometry:
   //       bits 0-15 reaction, 16-31 slot of species k in that reaction (0xFFFF: none), 32-39 net
   //       stoichiometric coefficient nu (int8); the gather coefficient is -nu * M_k
+  //   f64 tbe[n_tb]; i32 tbi[n_tb]; i32 tbs[n_tb]; f64 imw[ns]   the third-body lists (tb_eff, tb_idx, tb_slot)
+  //       and 1/M, read by the reactions phase (placed before exx_off, so the RHS-only kernel copies them too)
   //   u16 exx_off[(ns-1)*ns+1]; u32 exx[n_exx]   off-diagonal sensitivity entry (i, j), e = j*ns + i, <-
   //       slots of species j: bits 0-15 slot, 16-23 nu of species i (int8); coefficient -nu * M_i
   const char *tbl;
   int tbl_bytes;
   int sgx_at, exo_at, exx_at;  // byte offsets of sgx, exx_off and exx inside the table image
+  int tbe_at, tbi_at, tbs_at, imw_at;  // byte offsets of the third-body lists and 1/M
 };
 
 struct Species {

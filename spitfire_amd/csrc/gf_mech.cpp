@@ -231,6 +231,10 @@ int finalize(gf_mech *m) {
   };
   tput(sgx_off.data(), sgx_off.size() * sizeof(uint32_t));
   const int sgx_at = tput(sgx.data(), sgx.size() * sizeof(uint64_t));
+  const int tbe_at = tput(tb_eff.data(), tb_eff.size() * sizeof(double));
+  const int tbi_at = tput(tb_idx.data(), tb_idx.size() * sizeof(int));
+  const int tbs_at = tput(tb_slot.data(), tb_slot.size() * sizeof(int));
+  const int imw_at = tput(invmw.data(), invmw.size() * sizeof(double));
   const int exo_at = tput(exx_off.data(), exx_off.size() * sizeof(uint16_t));
   const int exx_at = tput(exx.data(), exx.size() * sizeof(uint32_t));
   tbl.resize((tbl.size() + 15) & ~size_t(15));
@@ -262,6 +266,7 @@ int finalize(gf_mech *m) {
   v.tbl = (const char *)P(a.put(tbl));
   v.tbl_bytes = (int)tbl.size();
   v.sgx_at = sgx_at; v.exo_at = exo_at; v.exx_at = exx_at;
+  v.tbe_at = tbe_at; v.tbi_at = tbi_at; v.tbs_at = tbs_at; v.imw_at = imw_at;
   m->blob.swap(a.buf);
   m->drx = drx;
   m->lds_per_state = sizeof(double) * (12 * NS_MAX + 16 + 5 * (size_t)nr + (size_t)nslot);

@@ -156,8 +156,8 @@ __global__ void __launch_bounds__(128) btd_twist_factor_kernel(const int *nblock
 
 // ---- Column-split variant: W waves share each block inverse ----
 //
-// The single-wave inverse above is a serial chain of N pivot steps, each about 300 instructions of one wave
-// (52 column updates, each two readlanes and two FP64 operations, plus the pivot search and the division),
+// The single-wave inverse above is a serial chain of N pivot steps, each a few hundred instructions of one
+// wave (at N = 53: 52 slot updates of two readlanes and an fma each, plus the pivot search and the reciprocal),
 // and a flamelet system has only one such chain per half. Splitting the columns of the block over W waves
 // divides the update work of a step by W: wave w holds slots w*C .. w*C + C - 1 (C = ceil(N / W)) of every
 // lane's row. Step j is pivoted by the wave that holds slot j; it publishes the multipliers g (one per lane;

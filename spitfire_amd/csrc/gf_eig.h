@@ -33,7 +33,7 @@ namespace gfa {
 // be an int or a compile-time constant (std::integral_constant), which lets the device fold row offsets into
 // the LDS instructions.
 template <class P, class LD>
-GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, LD ld, int *failed) {
+GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed) {
   using std::fabs;
   using std::sqrt;
   *failed = 0;
@@ -65,7 +65,9 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, LD ld, 
   }
 
   // 2) Householder reduction to upper Hessenberg form: column k is reduced below its subdiagonal by
-  //    H = I - beta v v^T acting on rows/columns k+1..n-1
+  //    H = I - beta v v^T acting on rows/columns k+1..n-1. The vector v is kept in column k itself (v0 in row
+  //    k+1, the other entries are the column's own), which neither update reads or writes, and column k is set
+  //    to alpha e1 after both: no separate vector buffer, so a matrix takes only its own n x ld words of LDS.
   for (int k = 0; k < n - 2; ++k) {
     const double x0 = a[(k + 1) * ld + k];
     const double tail = par.sum(k + 2, n, [&](int t) { const double x = a[t * ld + k]; return x * x; });
@@ -74,34 +76,31 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, LD ld, 
     const double alpha = x0 >= 0. ? -nrm : nrm;
     const double v0 = x0 - alpha;
     const double beta = -1. / (alpha * v0);
-    par.each(0, n, [&](int t) { vbuf[t] = t == k + 1 ? v0 : (t > k + 1 ? a[t * ld + k] : 0.); });
-    // left: rows k+1..n-1 of columns k+1..n-1 (column k becomes alpha e1 exactly); four rows per trip so that
-    // the loads of a trip are in flight together
+    par.each(k + 1, k + 2, [&](int t) { a[t * ld + k] = v0; });
+    const double *vk = a + k;  // v_i = vk[i * ld] for i > k
+    // left: rows k+1..n-1 of columns k+1..n-1; four rows per trip so that the loads of a trip are in flight
+    // together
     par.each(k + 1, n, [&](int j) {
       double w0 = 0., w1 = 0., w2 = 0., w3 = 0.;
       int i = k + 1;
       for (; i + 3 < n; i += 4) {
         const double x0 = a[i * ld + j], x1 = a[(i + 1) * ld + j], x2 = a[(i + 2) * ld + j], x3 = a[(i + 3) * ld + j];
-        w0 += vbuf[i] * x0;
-        w1 += vbuf[i + 1] * x1;
-        w2 += vbuf[i + 2] * x2;
-        w3 += vbuf[i + 3] * x3;
+        w0 += vk[i * ld] * x0;
+        w1 += vk[(i + 1) * ld] * x1;
+        w2 += vk[(i + 2) * ld] * x2;
+        w3 += vk[(i + 3) * ld] * x3;
       }
-      for (; i < n; ++i) w0 += vbuf[i] * a[i * ld + j];
+      for (; i < n; ++i) w0 += vk[i * ld] * a[i * ld + j];
       const double w = ((w0 + w1) + (w2 + w3)) * beta;
       i = k + 1;
       for (; i + 3 < n; i += 4) {
         const double x0 = a[i * ld + j], x1 = a[(i + 1) * ld + j], x2 = a[(i + 2) * ld + j], x3 = a[(i + 3) * ld + j];
-        a[i * ld + j] = x0 - vbuf[i] * w;
-        a[(i + 1) * ld + j] = x1 - vbuf[i + 1] * w;
-        a[(i + 2) * ld + j] = x2 - vbuf[i + 2] * w;
-        a[(i + 3) * ld + j] = x3 - vbuf[i + 3] * w;
+        a[i * ld + j] = x0 - vk[i * ld] * w;
+        a[(i + 1) * ld + j] = x1 - vk[(i + 1) * ld] * w;
+        a[(i + 2) * ld + j] = x2 - vk[(i + 2) * ld] * w;
+        a[(i + 3) * ld + j] = x3 - vk[(i + 3) * ld] * w;
       }
-      for (; i < n; ++i) a[i * ld + j] -= vbuf[i] * w;
-    });
-    par.each(0, n, [&](int t) {
-      if (t == k + 1) a[t * ld + k] = alpha;
-      else if (t > k + 1) a[t * ld + k] = 0.;
+      for (; i < n; ++i) a[i * ld + j] -= vk[i * ld] * w;
     });
     // right: all rows, columns k+1..n-1 (four columns per trip)
     par.each(0, n, [&](int i) {
@@ -110,22 +109,26 @@ GF_HD inline double eig_max_real(P &par, double *a, double *vbuf, int n, LD ld, 
       int j = k + 1;
       for (; j + 3 < n; j += 4) {
         const double x0 = ai[j], x1 = ai[j + 1], x2 = ai[j + 2], x3 = ai[j + 3];
-        u0 += x0 * vbuf[j];
-        u1 += x1 * vbuf[j + 1];
-        u2 += x2 * vbuf[j + 2];
-        u3 += x3 * vbuf[j + 3];
+        u0 += x0 * vk[j * ld];
+        u1 += x1 * vk[(j + 1) * ld];
+        u2 += x2 * vk[(j + 2) * ld];
+        u3 += x3 * vk[(j + 3) * ld];
       }
-      for (; j < n; ++j) u0 += ai[j] * vbuf[j];
+      for (; j < n; ++j) u0 += ai[j] * vk[j * ld];
       const double u = ((u0 + u1) + (u2 + u3)) * beta;
       j = k + 1;
       for (; j + 3 < n; j += 4) {
         const double x0 = ai[j], x1 = ai[j + 1], x2 = ai[j + 2], x3 = ai[j + 3];
-        ai[j] = x0 - u * vbuf[j];
-        ai[j + 1] = x1 - u * vbuf[j + 1];
-        ai[j + 2] = x2 - u * vbuf[j + 2];
-        ai[j + 3] = x3 - u * vbuf[j + 3];
+        ai[j] = x0 - u * vk[j * ld];
+        ai[j + 1] = x1 - u * vk[(j + 1) * ld];
+        ai[j + 2] = x2 - u * vk[(j + 2) * ld];
+        ai[j + 3] = x3 - u * vk[(j + 3) * ld];
       }
-      for (; j < n; ++j) ai[j] -= u * vbuf[j];
+      for (; j < n; ++j) ai[j] -= u * vk[j * ld];
+    });
+    par.each(0, n, [&](int t) {  // column k becomes alpha e1 exactly
+      if (t == k + 1) a[t * ld + k] = alpha;
+      else if (t > k + 1) a[t * ld + k] = 0.;
     });
   }
 

@@ -46,14 +46,13 @@ __global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ b
   extern __shared__ double smem[];
   const std::integral_constant<int, LD> ld{};  // row stride of the LDS matrix, odd and >= n
   double *a = smem;
-  double *vb = smem + (long)n * ld;
   WaveLanes par{(int)threadIdx.x};
   for (long b = blockIdx.x; b < nblk; b += gridDim.x) {
     const double *src = blocks + (boff ? boff[b] : b * bstride);
     for (int e = threadIdx.x; e < n * n; e += 64) a[(e % n) * ld + e / n] = src[e];
     __syncthreads();
     int failed = 0;
-    const double em = eig_max_real(par, a, vb, n, ld, &failed);
+    const double em = eig_max_real(par, a, n, ld, &failed);
     const double ex = fmax(em - diffterm, 0.);
     double *dst = out + (ooff ? ooff[b] : b * nout);
     for (int q = threadIdx.x; q < nout; q += 64) dst[q] = ex;
@@ -70,14 +69,14 @@ int eig_stride(int n) {
     if (s >= (n | 1)) return s;
   return 65;
 }
-size_t expeig_lds_bytes(int n) { return ((size_t)n * eig_stride(n) + 64) * sizeof(double); }
+size_t expeig_lds_bytes(int n) { return (size_t)n * eig_stride(n) * sizeof(double); }
 
 template <int LD>
 int launch_expeig_ld(const gf_mech *m, const double *blocks, const long *boff, const long *ooff, long nblk, int n,
                      double diffterm, double *out, int *nfail, hipStream_t st, int nout) {
   const size_t lds = expeig_lds_bytes(n);
-  // one wave per block, persistent: exactly as many workgroups as can be resident at once (LDS-limited, about
-  // 7 per CU at n = 53), each looping over its share of the blocks, so no workgroup waits for a second round
+  // one wave per block, persistent: exactly as many workgroups as can be resident at once (LDS-limited: 22.0 KiB
+  // per block at n = 53), each looping over its share of the blocks, so no workgroup waits for a second round
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, expeig_kernel<LD>, 64, lds) != hipSuccess || per_cu < 1)
     per_cu = 1;

@@ -4,9 +4,9 @@
 
 extern "C" double eig_max_real_host(const double *colmajor, int n, int *failed) {
   const int ld = n | 1;
-  std::vector<double> a((size_t)n * ld + 1), v(n + 1);
+  std::vector<double> a((size_t)n * ld + 1);
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) a[(size_t)i * ld + j] = colmajor[(size_t)j * n + i];
   gfa::HostLanes par;
-  return gfa::eig_max_real(par, a.data(), v.data(), n, ld, failed);
+  return gfa::eig_max_real(par, a.data(), n, ld, failed);
 }

@@ -44,8 +44,9 @@ GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed) {
   for (int sweep = 0; sweep < 64; ++sweep) {
     bool changed = false;
     for (int i = 0; i < n; ++i) {
-      double c = par.sum(0, n, [&](int t) { return t == i ? 0. : fabs(a[t * ld + i]); });
-      double r = par.sum(0, n, [&](int t) { return t == i ? 0. : fabs(a[i * ld + t]); });
+      // the norms only choose powers of 2, so single-precision sums serve (they are the latency-bound part)
+      double c = par.sumf(0, n, [&](int t) { return t == i ? 0.f : (float)fabs(a[t * ld + i]); });
+      double r = par.sumf(0, n, [&](int t) { return t == i ? 0.f : (float)fabs(a[i * ld + t]); });
       if (c == 0. || r == 0.) continue;
       double g = r * 0.5, f = 1.;
       const double s = c + r;
@@ -286,6 +287,11 @@ struct HostLanes {
   }
   template <class F> double sum(int lo, int hi, F f) {
     double s = 0.;
+    for (int t = lo; t < hi; ++t) s += f(t);
+    return s;
+  }
+  template <class F> float sumf(int lo, int hi, F f) {
+    float s = 0.f;
     for (int t = lo; t < hi; ++t) s += f(t);
     return s;
   }

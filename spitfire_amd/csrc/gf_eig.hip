@@ -32,6 +32,9 @@ struct WaveLanes {
     // DPP scan (no LDS permutes); the same value in every lane
     return wave::dpp_wave_sum((lane >= lo && lane < hi) ? f(lane) : 0.);
   }
+  template <class F> __device__ float sumf(int lo, int hi, F f) {
+    return wave::dpp_wave_sum_f32((lane >= lo && lane < hi) ? f(lane) : 0.f);
+  }
 };
 
 // expeig for block b: max(0, max Re(lambda(B_b)) - diffterm), written nout times to out[b*nout ..]

@@ -52,7 +52,8 @@ __global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ b
   WaveLanes par{(int)threadIdx.x};
   for (long b = blockIdx.x; b < nblk; b += gridDim.x) {
     const double *src = blocks + (boff ? boff[b] : b * bstride);
-    for (int e = threadIdx.x; e < n * n; e += 64) a[(e % n) * ld + e / n] = src[e];
+    if ((int)threadIdx.x < n)  // column j of the column-major block, lane = row: coalesced, no index division
+      for (int j = 0; j < n; ++j) a[threadIdx.x * ld + j] = src[(long)j * n + threadIdx.x];
     __syncthreads();
     int failed = 0;
     const double em = eig_max_real(par, a, n, ld, &failed);

@@ -254,13 +254,15 @@ __device__ __forceinline__ double wave_matvec(const T *D, int bs, double v, int 
   const bool act = lane < bs;
   double m0 = 0., m1 = 0., m2 = 0., m3 = 0.;
   int c = 0;
-  // FP64 blocks: sixteen columns per trip, their loads issued together before the products (column c still goes
-  // to the partial sum of c mod 4, in column order, so the result is the same as four at a time). Measured on the
-  // one-flamelet solve: FP64 0.518 -> 0.50 s; the FP32 blocks ran slower this way (0.44 -> 0.51 s) and keep four.
-  for (; std::is_same<T, double>::value && c + 15 < bs; c += 16) {
+  // Sixteen columns per trip, their loads issued together before the products (column c still goes to the
+  // partial sum of c mod 4, in column order, so the result is the same as four at a time). Lanes past the block
+  // read row 0 and discard it: a load under a lane condition was compiled as a branch around each load for
+  // FP32 blocks, which serialised them.
+  const int rl = act ? lane : 0;
+  for (; c + 15 < bs; c += 16) {
     double d[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) d[q] = act ? (double)D[(c + q) * bs + lane] : 0.;
+    for (int q = 0; q < 16; ++q) d[q] = (double)D[(c + q) * bs + rl];
 #pragma unroll
     for (int q = 0; q < 16; q += 4) {
       const double v0 = lane_value(v, c + q), v1 = lane_value(v, c + q + 1), v2 = lane_value(v, c + q + 2),
@@ -274,18 +276,21 @@ __device__ __forceinline__ double wave_matvec(const T *D, int bs, double v, int 
     }
   }
   for (; c + 3 < bs; c += 4) {
+    const double d0 = (double)D[c * bs + rl], d1 = (double)D[(c + 1) * bs + rl], d2 = (double)D[(c + 2) * bs + rl],
+                 d3 = (double)D[(c + 3) * bs + rl];
     const double v0 = lane_value(v, c), v1 = lane_value(v, c + 1), v2 = lane_value(v, c + 2),
                  v3 = lane_value(v, c + 3);
     if (act) {
-      m0 += (double)D[c * bs + lane] * v0;
-      m1 += (double)D[(c + 1) * bs + lane] * v1;
-      m2 += (double)D[(c + 2) * bs + lane] * v2;
-      m3 += (double)D[(c + 3) * bs + lane] * v3;
+      m0 += d0 * v0;
+      m1 += d1 * v1;
+      m2 += d2 * v2;
+      m3 += d3 * v3;
     }
   }
   for (; c < bs; ++c) {
+    const double d0 = (double)D[c * bs + rl];
     const double v0 = lane_value(v, c);
-    if (act) m0 += (double)D[c * bs + lane] * v0;
+    if (act) m0 += d0 * v0;
   }
   return (m0 + m1) + (m2 + m3);
 }

@@ -215,7 +215,7 @@ class ReactorBatch:
             self.lu.factor(M, st)
             ks = [f.clone()]
             X = [xn.clone()]
-            xs = xn.clone()
+            xs = X[0]  # read only until the next stage replaces it
             for s in range(1, 6):
                 prior.copy_(sum(A[s, j] * ks[j] for j in range(s)))
                 xi.copy_(xs)
@@ -251,7 +251,7 @@ class ReactorBatch:
                 its.zero_()
                 ks.append(fi.clone())
                 X.append(xi.clone())
-                xs = xi.clone()
+                xs = X[-1]
             dx = h[:, None] * sum(b[j] * ks[j] for j in range(6))
             err = torch.max(torch.abs(h[:, None] * sum((b[j] - bh[j]) * ks[j] for j in range(6)) * w), dim=1).values
             ok = torch.isfinite(dx).all(dim=1)

@@ -31,23 +31,44 @@ namespace gfa {
 
 // Matrix storage: row-major a[i * ld + j], ld odd (fewer LDS bank conflicts for column walks). The stride may
 // be an int or a compile-time constant (std::integral_constant), which lets the device fold row offsets into
-// the LDS instructions.
+// the LDS instructions. n is at most 64 (one lane per index); a larger n sets *failed and returns 0.
 template <class P, class LD>
 GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed) {
   using std::fabs;
   using std::sqrt;
-  *failed = 0;
-  if (n <= 0) return 0.;
+  *failed = n > 64;
+  if (n <= 0 || n > 64) return 0.;
   if (n == 1) return a[0];
 
-  // 1) balancing: scale row i by 1/f and column i by f (f a power of 2) until row and column norms agree
+  // 1) balancing: scale row i by 1/f and column i by f (f a power of 2) until row and column norms agree.
+  //    The off-diagonal column and row norms of every index are summed at the start of a sweep (each index
+  //    by its own lane, no reductions) and kept current as the sweep scales indices: scaling index i
+  //    multiplies a_ti, a term of row t's norm, by f and a_it, a term of column t's norm, by 1/f. The norms
+  //    only choose powers of 2, and every choice is an exact similarity, so neither their single precision
+  //    nor the rounding of the updates can change the eigenvalues; each sweep starts again from fresh sums.
+  typename P::LaneVal cn, rn;
   for (int sweep = 0; sweep < 64; ++sweep) {
+    par.each(0, n, [&](int t) {
+      float c0 = 0.f, c1 = 0.f, r0 = 0.f, r1 = 0.f;
+      int u = 0;
+      for (; u + 1 < n; u += 2) {
+        const double x0 = a[u * ld + t], x1 = a[(u + 1) * ld + t], y0 = a[t * ld + u], y1 = a[t * ld + u + 1];
+        c0 += u == t ? 0.f : (float)fabs(x0);
+        c1 += u + 1 == t ? 0.f : (float)fabs(x1);
+        r0 += u == t ? 0.f : (float)fabs(y0);
+        r1 += u + 1 == t ? 0.f : (float)fabs(y1);
+      }
+      if (u < n && u != t) {
+        c0 += (float)fabs(a[u * ld + t]);
+        r0 += (float)fabs(a[t * ld + u]);
+      }
+      par.at(cn, t) = c0 + c1;
+      par.at(rn, t) = r0 + r1;
+    });
     bool changed = false;
     for (int i = 0; i < n; ++i) {
-      // the norms only choose powers of 2, so single-precision sums serve (they are the latency-bound part)
-      double c = par.sumf(0, n, [&](int t) { return t == i ? 0.f : (float)fabs(a[t * ld + i]); });
-      double r = par.sumf(0, n, [&](int t) { return t == i ? 0.f : (float)fabs(a[i * ld + t]); });
-      if (c == 0. || r == 0.) continue;
+      double c = par.get(cn, i), r = par.get(rn, i);
+      if (!(c > 0.) || !(r > 0.)) continue;  // also a norm that cancellation left at or below zero
       double g = r * 0.5, f = 1.;
       const double s = c + r;
       int guard = 0;
@@ -57,9 +78,20 @@ GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed) {
       while (c > g && guard++ < 2100) { f *= 0.5; c *= 0.25; }
       if ((c + r) / f < 0.95 * s) {
         changed = true;
-        const double gi = 1. / f;
-        par.each(0, n, [&](int t) { a[i * ld + t] *= gi; });
-        par.each(0, n, [&](int t) { a[t * ld + i] *= f; });
+        const double gi = 1. / f, fm = f - 1., gm = gi - 1.;
+        par.each(0, n, [&](int t) {
+          float &ct = par.at(cn, t), &rt = par.at(rn, t);
+          if (t == i) {  // the diagonal entry is unchanged; the whole column grows by f, the row shrinks by it
+            ct = (float)(ct * f);
+            rt = (float)(rt * gi);
+          } else {
+            const double ait = a[i * ld + t], ati = a[t * ld + i];
+            a[i * ld + t] = ait * gi;
+            a[t * ld + i] = ati * f;
+            rt = (float)(rt + fabs(ati) * fm);
+            ct = (float)(ct + fabs(ait) * gm);
+          }
+        });
       }
     }
     if (!changed) break;
@@ -277,6 +309,12 @@ GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed) {
 
 // Host policy: lane loops run sequentially (used by the CPU unit tests of this file).
 struct HostLanes {
+  // one value per index (the device keeps index t's in lane t's register)
+  struct LaneVal {
+    float v[64];
+  };
+  float &at(LaneVal &x, int t) { return x.v[t]; }
+  float get(const LaneVal &x, int i) { return x.v[i]; }
   template <class F> void each(int lo, int hi, F f) { for (int t = lo; t < hi; ++t) f(t); }
   double recip(double x) { return 1. / x; }
   // the largest i in [lo, hi) with pred(i), or lo - 1 if none
@@ -287,11 +325,6 @@ struct HostLanes {
   }
   template <class F> double sum(int lo, int hi, F f) {
     double s = 0.;
-    for (int t = lo; t < hi; ++t) s += f(t);
-    return s;
-  }
-  template <class F> float sumf(int lo, int hi, F f) {
-    float s = 0.f;
     for (int t = lo; t < hi; ++t) s += f(t);
     return s;
   }

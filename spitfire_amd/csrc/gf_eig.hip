@@ -11,6 +11,12 @@ namespace gfa {
 // Device policy for eig_max_real: one 64-lane wave per matrix, lane t owns row/column t.
 struct WaveLanes {
   int lane;
+  // one value per index, held by the index's lane
+  struct LaneVal {
+    float v;
+  };
+  __device__ float &at(LaneVal &x, int) { return x.v; }
+  __device__ float get(const LaneVal &x, int i) { return wave::lane_value(x.v, i); }
   template <class F> __device__ void each(int lo, int hi, F f) {
     if (lane >= lo && lane < hi) f(lane);
     __syncthreads();
@@ -31,9 +37,6 @@ struct WaveLanes {
   template <class F> __device__ double sum(int lo, int hi, F f) {
     // DPP scan (no LDS permutes); the same value in every lane
     return wave::dpp_wave_sum((lane >= lo && lane < hi) ? f(lane) : 0.);
-  }
-  template <class F> __device__ float sumf(int lo, int hi, F f) {
-    return wave::dpp_wave_sum_f32((lane >= lo && lane < hi) ? f(lane) : 0.f);
   }
 };
 

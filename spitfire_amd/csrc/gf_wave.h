@@ -36,22 +36,6 @@ __device__ __forceinline__ double dpp_wave_sum(double v) {
                           __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
 
-// Sum of a float over the 64 lanes, returned to every lane (the same DPP scan in single precision: each step is
-// one add with a DPP operand).
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ float dpp_add_f32(float v) {
-  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xF, false));
-}
-__device__ __forceinline__ float dpp_wave_sum_f32(float v) {
-  v = dpp_add_f32<0x111, 0xF>(v);  // row_shr:1
-  v = dpp_add_f32<0x112, 0xF>(v);  // row_shr:2
-  v = dpp_add_f32<0x114, 0xF>(v);  // row_shr:4
-  v = dpp_add_f32<0x118, 0xF>(v);  // row_shr:8
-  v = dpp_add_f32<0x142, 0xA>(v);  // row_bcast:15
-  v = dpp_add_f32<0x143, 0xC>(v);  // row_bcast:31
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-}
-
 // First lane holding the largest a >= 0 among the candidate lanes (LAPACK idamax semantics); -1 if none.
 // Non-negative doubles order like their bit patterns, so the maximum is found on the high word, then on the
 // low word among the lanes that hold the maximal high word (skipped when only one lane holds it).

@@ -32,9 +32,15 @@ namespace gfa {
 // Matrix storage: row-major a[i * ld + j], ld odd (fewer LDS bank conflicts for column walks). The stride may
 // be an int or a compile-time constant (std::integral_constant), which lets the device fold row offsets into
 // the LDS instructions. n is at most 64 (one lane per index); a larger n sets *failed and returns 0.
+//
+// kexp, when given, holds n balancing exponents (floats holding integers) carried from one call to the next
+// on a similar matrix: the matrix is first scaled by them (a_tu * 2^(k_u - k_t), an exact similarity), the
+// balancing sweeps continue from there, and the exponents of the final scaling are written back. Any
+// exponents give the same eigenvalues in exact arithmetic; good ones save most of the sweeps.
 template <class P, class LD>
-GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed) {
+GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed, float *kexp = nullptr) {
   using std::fabs;
+  using std::ldexp;
   using std::sqrt;
   *failed = n > 64;
   if (n <= 0 || n > 64) return 0.;
@@ -46,7 +52,13 @@ GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed) {
   //    multiplies a_ti, a term of row t's norm, by f and a_it, a term of column t's norm, by 1/f. The norms
   //    only choose powers of 2, and every choice is an exact similarity, so neither their single precision
   //    nor the rounding of the updates can change the eigenvalues; each sweep starts again from fresh sums.
-  typename P::LaneVal cn, rn;
+  typename P::LaneVal cn, rn, ke;
+  par.each(0, n, [&](int t) { par.at(ke, t) = kexp ? kexp[t] : 0.f; });
+  if (kexp)
+    par.each(0, n, [&](int t) {
+      const int kt = (int)par.at(ke, t);
+      for (int u = 0; u < n; ++u) a[t * ld + u] = ldexp(a[t * ld + u], (int)par.get(ke, u) - kt);
+    });
   for (int sweep = 0; sweep < 64; ++sweep) {
     par.each(0, n, [&](int t) {
       float c0 = 0.f, c1 = 0.f, r0 = 0.f, r1 = 0.f;
@@ -71,11 +83,11 @@ GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed) {
       if (!(c > 0.) || !(r > 0.)) continue;  // also a norm that cancellation left at or below zero
       double g = r * 0.5, f = 1.;
       const double s = c + r;
-      int guard = 0;
-      while (c < g && guard++ < 2100) { f *= 2.; c *= 4.; }
+      int guard = 0, k = 0;
+      while (c < g && guard++ < 2100) { f *= 2.; c *= 4.; ++k; }
       g = r * 2.;
       guard = 0;
-      while (c > g && guard++ < 2100) { f *= 0.5; c *= 0.25; }
+      while (c > g && guard++ < 2100) { f *= 0.5; c *= 0.25; --k; }
       if ((c + r) / f < 0.95 * s) {
         changed = true;
         const double gi = 1. / f, fm = f - 1., gm = gi - 1.;
@@ -84,6 +96,7 @@ GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed) {
           if (t == i) {  // the diagonal entry is unchanged; the whole column grows by f, the row shrinks by it
             ct = (float)(ct * f);
             rt = (float)(rt * gi);
+            par.at(ke, t) += (float)k;
           } else {
             const double ait = a[i * ld + t], ati = a[t * ld + i];
             a[i * ld + t] = ait * gi;
@@ -95,6 +108,14 @@ GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed) {
       }
     }
     if (!changed) break;
+  }
+  // Exponents are carried only while all of them stay within +-96; otherwise the next call starts cold. Chemistry
+  // blocks with entries over ~280 decades can end with exponents near 200: started from those, the next
+  // refresh's slightly different (nearly reducible) block balanced to a far larger norm, and its eigenvalues lost
+  // all accuracy, while no block started from exponents within +-128 did.
+  if (kexp) {
+    const bool keep = par.last_true(0, n, [&](int t) { return fabs(par.at(ke, t)) > 96.f; }) < 0;
+    par.each(0, n, [&](int t) { kexp[t] = keep ? par.at(ke, t) : 0.f; });
   }
 
   // 2) Householder reduction to upper Hessenberg form: column k is reduced below its subdiagonal by

@@ -43,12 +43,13 @@ struct WaveLanes {
 // expeig for block b: max(0, max Re(lambda(B_b)) - diffterm), written nout times to out[b*nout ..]
 // (flamelet_kernels.cpp:1329-1341, which writes it once per point unknown: nout = n). Blocks are column-major
 // n x n at blocks + b*bstride. With boff/ooff given, block b starts at blocks + boff[b] and its outputs at
-// out + ooff[b].
+// out + ooff[b]. kexp, when given, is laid out like out with nout = n and carries each block's balancing
+// exponents from one launch to the next.
 template <int LD>
 __global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ blocks, long bstride,
                                                     const long *__restrict__ boff, const long *__restrict__ ooff,
                                                     long nblk, int n, double diffterm, double *__restrict__ out,
-                                                    int nout, int *__restrict__ nfail) {
+                                                    int nout, int *__restrict__ nfail, float *__restrict__ kexp) {
   extern __shared__ double smem[];
   const std::integral_constant<int, LD> ld{};  // row stride of the LDS matrix, odd and >= n
   double *a = smem;
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(64) void expeig_kernel(const double *__restrict__ b
       for (int j = 0; j < n; ++j) a[threadIdx.x * ld + j] = src[(long)j * n + threadIdx.x];
     __syncthreads();
     int failed = 0;
-    const double em = eig_max_real(par, a, n, ld, &failed);
+    const double em = eig_max_real(par, a, n, ld, &failed, kexp ? kexp + (ooff ? ooff[b] : b * n) : nullptr);
     const double ex = fmax(em - diffterm, 0.);
     double *dst = out + (ooff ? ooff[b] : b * nout);
     for (int q = threadIdx.x; q < nout; q += 64) dst[q] = ex;
@@ -80,7 +81,7 @@ size_t expeig_lds_bytes(int n) { return (size_t)n * eig_stride(n) * sizeof(doubl
 
 template <int LD>
 int launch_expeig_ld(const gf_mech *m, const double *blocks, const long *boff, const long *ooff, long nblk, int n,
-                     double diffterm, double *out, int *nfail, hipStream_t st, int nout) {
+                     double diffterm, double *out, int *nfail, hipStream_t st, int nout, float *kexp) {
   const size_t lds = expeig_lds_bytes(n);
   // one wave per block, persistent: exactly as many workgroups as can be resident at once (LDS-limited: 22.0 KiB
   // per block at n = 53), each looping over its share of the blocks, so no workgroup waits for a second round
@@ -90,26 +91,27 @@ int launch_expeig_ld(const gf_mech *m, const double *blocks, const long *boff, c
   long grid = (long)(m && m->n_cu > 0 ? m->n_cu : 256) * per_cu;
   if (grid > nblk) grid = nblk;
   hipLaunchKernelGGL(expeig_kernel<LD>, dim3((unsigned)grid), dim3(64), lds, st, blocks, (long)n * n, boff, ooff, nblk,
-                     n, diffterm, out, nout, nfail);
+                     n, diffterm, out, nout, nfail, kexp);
   return hipGetLastError() == hipSuccess ? GF_OK : fail(GF_ERR_HIP, "eigenvalue kernel launch failed");
 }
 
 int launch_expeig_list(const gf_mech *m, const double *blocks, const long *boff, const long *ooff, long nblk, int n,
-                       double diffterm, double *out, int *nfail, hipStream_t st, int nout) {
+                       double diffterm, double *out, int *nfail, hipStream_t st, int nout, float *kexp) {
   if (nout < 0) nout = n;
   if (nblk <= 0) return GF_OK;
   if (n < 1 || n > 64) return fail(GF_ERR_UNSUPPORTED, "eigenvalue blocks must be 1..64 wide");
+  if (kexp && nout != n) return fail(GF_ERR_ARG, "carried balancing exponents need one output per block row");
   switch (eig_stride(n)) {
-    case 9: return launch_expeig_ld<9>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
-    case 17: return launch_expeig_ld<17>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
-    case 25: return launch_expeig_ld<25>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
-    case 33: return launch_expeig_ld<33>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
-    case 41: return launch_expeig_ld<41>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
-    case 49: return launch_expeig_ld<49>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
-    case 53: return launch_expeig_ld<53>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
-    case 57: return launch_expeig_ld<57>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
-    case 61: return launch_expeig_ld<61>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
-    default: return launch_expeig_ld<65>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout);
+    case 9: return launch_expeig_ld<9>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout, kexp);
+    case 17: return launch_expeig_ld<17>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout, kexp);
+    case 25: return launch_expeig_ld<25>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout, kexp);
+    case 33: return launch_expeig_ld<33>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout, kexp);
+    case 41: return launch_expeig_ld<41>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout, kexp);
+    case 49: return launch_expeig_ld<49>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout, kexp);
+    case 53: return launch_expeig_ld<53>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout, kexp);
+    case 57: return launch_expeig_ld<57>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout, kexp);
+    case 61: return launch_expeig_ld<61>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout, kexp);
+    default: return launch_expeig_ld<65>(m, blocks, boff, ooff, nblk, n, diffterm, out, nfail, st, nout, kexp);
   }
 }
 

@@ -61,6 +61,7 @@ struct gf_flamelet_batch {
          *cmaj = nullptr, *csub = nullptr, *csup = nullptr, *mco = nullptr, *nco = nullptr, *chi = nullptr,
          *dz = nullptr, *dp = nullptr, *fval = nullptr, *fres = nullptr, *fmin = nullptr, *fdiff = nullptr;
   float *D32 = nullptr;  // FP32 inverse diagonal blocks (mixed-precision path); L holds the FP64 ones
+  float *ekx = nullptr;  // [NV] balancing exponents of each point's chemistry block, carried between refreshes
   double *hl = nullptr;  // heat-loss parameters [4][NI] (own allocation), null while adiabatic
   double *ile = nullptr; // inverse Lewis numbers [B][ns] (own allocation), null for unity
   int *piv = nullptr, *dnzi = nullptr, *flag = nullptr, *gmin = nullptr, *sel = nullptr, *sys_nb = nullptr;
@@ -100,6 +101,7 @@ size_t carve_all(gf_flamelet_batch *b, char *base) {
   b->dX = c.take<double>(b->NV);
   b->ds = c.take<double>(b->NV);
   b->eig = c.take<double>(b->NV);
+  b->ekx = c.take<float>(b->NV);
   b->J = c.take<double>(b->NJ);
   b->L = c.take<double>(b->NL);
   b->D32 = c.take<float>(b->NL);
@@ -273,8 +275,10 @@ struct Psitc {
       }
     GF_TRY(up(b->eb, eb));
     GF_TRY(up(b->eo, eo));
-    // max(0, max Re(lambda)); the per-flamelet diffusion shift is applied with the step update
-    GF_TRY(launch_expeig_list(b->m, b->J, b->eb, b->eo, (long)eb.size(), b->ns, 0., b->eig, nullptr, st));
+    // max(0, max Re(lambda)); the per-flamelet diffusion shift is applied with the step update. Each point's
+    // balancing starts from the scaling its block had at the previous refresh (the same eigenvalues, fewer sweeps).
+    GF_TRY(launch_expeig_list(b->m, b->J, b->eb, b->eo, (long)eb.size(), b->ns, 0., b->eig, nullptr, st, b->ns,
+                              b->ekx));
     GF_TRY(up(b->flag, flags_of(fl)));
     std::vector<int> g(b->B, 0);
     for (size_t k = 0; k < fl.size(); ++k) g[fl[k]] = first[k];
@@ -571,6 +575,7 @@ int gf_flamelet_batch_create(gf_mech *m, int nfl, const int *nzi, double pressur
   T(up(b->dchi0, b->chi0, st));
   T(up(b->dvoff, b->voff, st));
   T(up(b->ddz0, b->dz0, st));
+  T(up(b->ekx, std::vector<float>((size_t)V, 0.f), st));
   FlameletBatchView &v = b->view;
   v.B = nfl;
   v.ns = ns;

@@ -261,8 +261,10 @@ int launch_fb_ds(const FlameletBatchView &b, const int *flag, const int *global_
 int launch_fb_fill_dofs(const FlameletBatchView &b, const int *flag, const double *value, double *dst,
                         hipStream_t st);
 int launch_fb_clip_points(long n, double *X, hipStream_t st);
+// kexp (optional, laid out like out with nout = n) carries each block's balancing exponents between launches
 int launch_expeig_list(const gf_mech *m, const double *blocks, const long *boff, const long *ooff, long nblk, int n,
-                       double diffterm, double *out, int *nfail, hipStream_t st, int nout = -1);
+                       double diffterm, double *out, int *nfail, hipStream_t st, int nout = -1,
+                       float *kexp = nullptr);
 // block-tridiagonal kernels (gf_btd.hip)
 int launch_btd_factor(int nsys, const int *nblocks, int bs, double *values, const long *voff, double *l_values,
                       const long *loff, int *pivots, const long *poff, hipStream_t st);

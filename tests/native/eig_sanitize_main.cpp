@@ -15,13 +15,13 @@ static double uniform() {  // xorshift64*
   return (double)((g_state * 2685821657736338717ull) >> 11) * (1.0 / 9007199254740992.0);
 }
 
-static double run(const std::vector<double> &colmajor, int n, int *failed) {
+static double run(const std::vector<double> &colmajor, int n, int *failed, float *kexp = nullptr) {
   const int ld = n | 1;
   std::vector<double> a((size_t)n * ld + 1);
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) a[(size_t)i * ld + j] = colmajor[(size_t)j * n + i];
   gfa::HostLanes par;
-  return gfa::eig_max_real(par, a.data(), n, ld, failed);
+  return gfa::eig_max_real(par, a.data(), n, ld, failed, kexp);
 }
 
 int main() {
@@ -38,12 +38,18 @@ int main() {
           if (kind == 4) x = (i == j + 1) ? 1. : (j == n - 1 ? -(double)(i + 1) : 0.);  // companion
           A[(size_t)j * n + i] = x;
         }
-      int failed = 0;
-      const double r = run(A, n, &failed);
-      ++cases;
-      if (!std::isfinite(r) || failed) {
-        ++bad;
-        std::printf("n=%d kind=%d: result %g, failed %d\n", n, kind, r, failed);
+      // plain, then with carried balancing exponents: zero, the ones written back, then arbitrary ones
+      std::vector<float> k((size_t)n, 0.f);
+      for (int pass = 0; pass < 4; ++pass) {
+        if (pass == 3)
+          for (int i = 0; i < n; ++i) k[(size_t)i] = (float)(int)(80. * uniform() - 40.);
+        int failed = 0;
+        const double r = run(A, n, &failed, pass ? k.data() : nullptr);
+        ++cases;
+        if (!std::isfinite(r) || failed) {
+          ++bad;
+          std::printf("n=%d kind=%d pass=%d: result %g, failed %d\n", n, kind, pass, r, failed);
+        }
       }
     }
   }

@@ -24,6 +24,8 @@ def eiglib():
     lib = ctypes.CDLL(so)
     lib.eig_max_real_host.restype = ctypes.c_double
     lib.eig_max_real_host.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.eig_max_real_host_k.restype = ctypes.c_double
+    lib.eig_max_real_host_k.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
     return lib
 
 
@@ -31,6 +33,13 @@ def _emax(lib, A):
     A = np.asfortranarray(A, dtype=np.float64)
     f = ctypes.c_int()
     r = lib.eig_max_real_host(A.ctypes.data, A.shape[0], ctypes.byref(f))
+    return r, f.value
+
+
+def _emax_k(lib, A, k):
+    A = np.asfortranarray(A, dtype=np.float64)
+    f = ctypes.c_int()
+    r = lib.eig_max_real_host_k(A.ctypes.data, A.shape[0], ctypes.byref(f), k.ctypes.data)
     return r, f.value
 
 
@@ -83,3 +92,27 @@ def test_flamelet_chemistry_blocks(eiglib, fx, mech, request, oracle_lib):
     pre = o.last_chemistry_blocks
     for iz in range(nzi):
         _check(eiglib, pre[iz * ns * ns:(iz + 1) * ns * ns].reshape((ns, ns), order='F'))
+
+
+def test_carried_balancing_exponents(eiglib):
+    """Balancing may start from exponents carried over from an earlier, similar matrix. Any exponents are an
+    exact power-of-2 similarity, so the eigenvalues must stay those of dgeev. Exponents from a converged
+    balancing of the same matrix must come back unchanged with a bitwise identical result (the scaled matrix is
+    the balanced one bit for bit, so no sweep changes anything)."""
+    rng = np.random.default_rng(7)
+    for n in (3, 10, 31, 53, 64):
+        for trial in range(6):
+            A = rng.standard_normal((n, n)) * 10. ** rng.uniform(-4, 4, (n, 1)) * 10. ** rng.uniform(-4, 4, (1, n))
+            lam = np.linalg.eigvals(A)
+            tol = 1e-12 * np.abs(lam).max()
+            k = np.zeros(n, np.float32)
+            cold, failed = _emax_k(eiglib, A, k)
+            assert failed == 0 and abs(cold - lam.real.max()) <= tol
+            assert cold == _emax(eiglib, A)[0]  # zero exponents are the plain routine
+            k1 = k.copy()
+            again, failed = _emax_k(eiglib, A, k1)
+            assert failed == 0 and again == cold and np.array_equal(k1, k)
+            kr = rng.integers(-40, 41, n).astype(np.float32)  # arbitrary exponents: same eigenvalues
+            r, failed = _emax_k(eiglib, A, kr)
+            assert failed == 0 and abs(r - lam.real.max()) <= tol, (n, trial, r, lam.real.max())
+            assert np.all(kr == np.round(kr)) and np.all(np.abs(kr) <= 96)

@@ -34,9 +34,10 @@ namespace gfa {
 // the LDS instructions. n is at most 64 (one lane per index); a larger n sets *failed and returns 0.
 //
 // kexp, when given, holds n balancing exponents (floats holding integers) carried from one call to the next
-// on a similar matrix: the matrix is first scaled by them (a_tu * 2^(k_u - k_t), an exact similarity), the
-// balancing sweeps continue from there, and the exponents of the final scaling are written back. Any
-// exponents give the same eigenvalues in exact arithmetic; good ones save most of the sweeps.
+// on a similar matrix: the matrix is first scaled by them (a_tu * 2^(k_u - k_t), a similarity that is exact
+// unless an entry underflows), the balancing sweeps continue from there, and the exponents of the final
+// scaling are written back. Any exponents give the same eigenvalues in exact arithmetic; good ones save most
+// of the sweeps.
 template <class P, class LD>
 GF_HD inline double eig_max_real(P &par, double *a, int n, LD ld, int *failed, float *kexp = nullptr) {
   using std::fabs;

@@ -213,13 +213,13 @@ class ReactorBatch:
             self._jac(xn, f, J)
             torch.sub(hb[:, None, None] * g * J.view(R, ns, ns), eye, out=M.view(R, ns, ns))
             self.lu.factor(M, st)
-            ks = [f.clone()]
+            ks = [f]  # f is not written again within the step
             X = [xn.clone()]
-            xs = X[0]  # read only until the next stage replaces it
             for s in range(1, 6):
                 prior.copy_(sum(A[s, j] * ks[j] for j in range(s)))
-                xi.copy_(xs)
-                fi.copy_(ks[-1])
+                if s == 1:  # later stages start from the previous stage's solution, which xi and fi still hold
+                    xi.copy_(xn)
+                    fi.copy_(f)
                 conv.zero_()
                 # stage residual r = h (gamma f + prior) - (x - xn), then Newton: x -= solve(r) for the
                 # reactors not yet converged, f(x), residual and convergence test (fused kernels)
@@ -251,7 +251,6 @@ class ReactorBatch:
                 its.zero_()
                 ks.append(fi.clone())
                 X.append(xi.clone())
-                xs = X[-1]
             dx = h[:, None] * sum(b[j] * ks[j] for j in range(6))
             err = torch.max(torch.abs(h[:, None] * sum((b[j] - bh[j]) * ks[j] for j in range(6)) * w), dim=1).values
             ok = torch.isfinite(dx).all(dim=1)

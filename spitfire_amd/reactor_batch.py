@@ -337,7 +337,7 @@ class ReactorBatch:
         grad = torch.zeros((R, nr), dtype=torch.float64, device=dev)
         L = torch.zeros(R, dtype=torch.float64, device=dev)
         st = self._stream()
-        th = torch.empty((R, ns), dtype=torch.float64, device=dev)
+        thb = torch.empty((6, R, ns), dtype=torch.float64, device=dev)  # each stage's theta, one step at a time
         Mt = torch.empty((R, ns * ns), dtype=torch.float64, device=dev)
         rhs = torch.empty((R, ns), dtype=torch.float64, device=dev)
         cm = torch.empty((R, ns), dtype=torch.float64, device=dev)
@@ -355,13 +355,14 @@ class ReactorBatch:
                     if A[j, i] != 0.:
                         v = v + A[j, i] * thetas[j]
                 v = v.contiguous()
-                # rhs = h (J^T v + b_i grad qdot) and, for an implicit stage, I - h a_ii J^T
-                self._adjoint_prep(x, f, J, aux, cpi, dinvm, v, h, b[i], A[i, i], rhs, Mt if i > 0 else None)
+                # rhs = h (J^T v + b_i grad qdot) and, for an implicit stage, I - h a_ii J^T; the explicit first
+                # stage's rhs is its theta
+                theta = thb[i]
+                self._adjoint_prep(x, f, J, aux, cpi, dinvm, v, h, b[i], A[i, i], rhs if i > 0 else theta,
+                                   Mt if i > 0 else None)
                 if i > 0:
                     self.lu.factor(Mt, st)
-                    theta = self.lu.solve(rhs, th, st).clone()
-                else:
-                    theta = rhs.clone()
+                    self.lu.solve(rhs, theta, st)
                 thetas[i] = theta
                 self._adjoint_finish(v, theta, aux, hs, f, h, b[i], A[i, i], cm, L)
                 grad += h[:, None] * qa * (cm @ self.N)
